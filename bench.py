@@ -1,0 +1,211 @@
+#!/usr/bin/env python3
+"""Benchmark of the BCGS row-slab orthogonalization on one MI355X.
+
+Workload (one step = one full orthogonalization, inputs resident in HBM):
+BCGSI+P-1S in FP64 on a row slab X of m = 2^24 rows, n = 256 columns,
+block width s = 16 (16 blocks), X = U diag(sigma) V^T with sigma log-spaced
+over kappa = 1e6 (seeded, generated on the device).
+
+Prints one JSON line.  `value` is the achieved HBM bandwidth on the
+algorithmic byte count B_alg = 8 m (2n + s N(N-1)/2) of one factorization
+(read X once, write Q once, re-read the Q prefix once per block) divided by
+the HIP-event time of the call.  `roofline` describes the dominant kernel
+class, timed with HIP events around each of its launches on the stream it
+runs on; `cpu_baseline` times the CPU oracle (a C restatement of the
+reference) on a bounded sample of the same workload, scaled linearly in m.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for sub in ("one-synchronization-bcgs_amd", "oracle", "tests"):
+    p = os.path.join(ROOT, sub)
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+METRIC = "BCGSI+P-1S ms & HBM GB/s, m=2^24 n=256 s=16: mixed vs FP64 at equal LOO"
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def b_alg(m, n, s, elem=8):
+    N = 1 + (n - s + s - 1) // s
+    return elem * m * (2 * n + s * N * (N - 1) // 2)
+
+
+def f_alg(m, n):
+    return 4.0 * m * n * n
+
+
+def make_input(torch, m, n, kappa, seed):
+    """X^T as an n x m row-major tensor (= column-major m x n, ld m)."""
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    G = torch.randn((n, m), generator=g, dtype=torch.float64, device="cuda")
+    for _ in range(2):  # CholQR2: rows of G become orthonormal (U^T)
+        L = torch.linalg.cholesky(G @ G.T)
+        G = torch.linalg.inv(L) @ G
+    V, _ = torch.linalg.qr(torch.randn((n, n), generator=g, dtype=torch.float64,
+                                       device="cuda"))
+    sig = torch.logspace(0, -float(torch.log10(torch.tensor(kappa))), n,
+                         dtype=torch.float64, device="cuda")
+    X = (V * sig) @ G
+    del G
+    return X.contiguous()
+
+
+def cpu_baseline(n, s, kappa, rows, threads, m_full):
+    import numpy as np
+
+    import inputs
+    import oracle
+
+    oracle.set_threads(threads)
+    X = inputs.svd_matrix(rows, n, kappa, seed=1)
+    t0 = time.perf_counter()
+    Q, R, info = oracle.bcgs("bcgsi2P1s", X, s)
+    t = time.perf_counter() - t0
+    loo = inputs.loo(Q)
+    t_full = t * (m_full / rows)
+    return {
+        "value": round(b_alg(m_full, n, s) / t_full / 1e9, 3),
+        "unit": "GB/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": ("CPU oracle bcgsi2P1s FP64 on m=%d rows (n=%d, s=%d, kappa=%g), "
+                   "%.2f s, scaled linearly in m to m=%d: %.1f ms per factorization; "
+                   "LOO %.1e" % (rows, n, s, kappa, t, m_full, t_full * 1e3, loo)),
+        "ms_per_factorization": round(t_full * 1e3, 1),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--m", type=int, default=1 << 24)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--s", type=int, default=16)
+    ap.add_argument("--kappa", type=float, default=1e6)
+    ap.add_argument("--variant", default="bcgsi2P1s")
+    ap.add_argument("--cpu-rows", type=int, default=1 << 19)
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-accuracy", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: the row slab of one rank at its real "
+              "shape; multi-GPU runs are not part of this build (--gpus 1).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import bcgs_hip as B
+
+    B.lib()
+    m, n, s = args.m, args.n, args.s
+    log("generating X: m=%d n=%d kappa=%g" % (m, n, args.kappa))
+    X0 = make_input(torch, m, n, args.kappa, seed=1)
+    X = torch.empty_like(X0)
+    R = torch.zeros(n * n, dtype=torch.float64, device="cuda")
+    stream = torch.cuda.current_stream()
+    sp = stream.cuda_stream
+
+    def step():
+        B.bcgs_device(args.variant, m, n, s, X.data_ptr(), m, R.data_ptr(), n, sp)
+
+    for _ in range(args.warmup):
+        X.copy_(X0)
+        step()
+    torch.cuda.synchronize()
+    log("warmup done")
+
+    B.kernel_timing(True)
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        X.copy_(X0)  # restore the input; outside the event-timed region
+        evs[k][0].record(stream)
+        step()
+        evs[k][1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kt = B.kernel_times()
+    B.kernel_timing(False)
+    step_ms = [a.elapsed_time(b) for a, b in evs]
+    ms = sum(step_ms) / len(step_ms)
+    st = B.status()
+    log("steps ms:", ["%.2f" % v for v in step_ms], "wall %.3f s" % wall)
+    log("kernel classes:", json.dumps(kt))
+
+    acc = {}
+    if not args.no_accuracy:
+        loo, res, xn = B.accuracy_device(m, n, X0.data_ptr(), m, X.data_ptr(), m,
+                                         R.data_ptr(), n, sp)
+        acc = {"loo": loo, "rel_residual": res / xn}
+        log("LOO %.3e  residual %.3e" % (loo, res / xn))
+
+    # Dominant kernel class and its roofline.
+    dom = max(kt.items(), key=lambda kv: kv[1]["ms"])
+    name, d = dom
+    sec = d["ms"] / 1e3
+    gbs = d["bytes"] / sec / 1e9
+    tfs = d["flops"] / sec / 1e12
+    if d["flops"] / (FP64_PEAK_TFLOPS * 1e12) >= d["bytes"] / (HBM_PEAK_GBS * 1e9):
+        roof = {"bound": "mfma", "achieved": round(tfs, 3), "peak": FP64_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": round(tfs / FP64_PEAK_TFLOPS, 4)}
+    else:
+        roof = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    roof.update({"traffic": None, "kernel": name, "launches": d["launches"],
+                 "avg_launch_ms": round(d["ms"] / d["launches"], 4),
+                 "share_of_step": round(d["ms"] / (ms * args.steps), 3),
+                 "alg_gbs": round(gbs, 1), "alg_tflops": round(tfs, 3)})
+
+    value = b_alg(m, n, s) / (ms / 1e3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded U diag(sigma) V^T generated on the device)",
+        "config": {"workload": "%s FP64 m=%d n=%d s=%d kappa=%g" % (args.variant, m, n, s,
+                                                                     args.kappa),
+                   "variant": args.variant, "m": m, "n": n, "s": s, "kappa": args.kappa},
+        "fp64_tflops_alg": round(f_alg(m, n) / (ms / 1e3) / 1e12, 3),
+        "frac_hbm_peak": round(value / HBM_PEAK_GBS, 4),
+        "frac_fp64_peak": round(f_alg(m, n) / (ms / 1e3) / 1e12 / FP64_PEAK_TFLOPS, 4),
+        "passes": st["passes"],
+        "syncs": st["syncs"],
+        "info_block": st["info_block"],
+        "roofline": roof,
+    }
+    out.update(acc)
+    if not args.no_cpu:
+        log("CPU baseline ...")
+        out["cpu_baseline"] = cpu_baseline(n, s, args.kappa, args.cpu_rows,
+                                           args.cpu_threads, m)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

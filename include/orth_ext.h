@@ -1,0 +1,111 @@
+/*
+ * Extended interface of liborth_hip.so: device-pointer and stream entry
+ * points, status reporting, the accuracy harness, the mixed-precision form
+ * and the batched form.  The five reference-compatible symbols are in
+ * orth.h; nothing here has a counterpart in the reference's include/orth.h
+ * unless its comment cites one.
+ *
+ * Conventions: matrices are column-major FP64 unless stated; sizes are
+ * 64-bit; `stream` is a hipStream_t passed as void* (NULL = null stream).
+ * Device entry points enqueue work and return without waiting; successive
+ * calls on one device are ordered by the library even across streams,
+ * because they share the library's device scratch.
+ */
+#ifndef ORTH_EXT_H
+#define ORTH_EXT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Variant ids. */
+enum {
+    ORTH_P1S = 0,  /* bcgsi2P1s  */
+    ORTH_1S = 1,   /* bcgsi21s   */
+    ORTH_PIPI = 2, /* bcgspipi2  */
+    ORTH_I2 = 3,   /* bcgsi2     */
+    ORTH_P2S = 4   /* bcgsi2P2s  */
+};
+
+/* Precision of the bulk products in the mixed form. */
+enum {
+    ORTH_PREC_FP64 = 0,   /* everything FP64 (the drop-in arithmetic)        */
+    ORTH_PREC_BF16 = 1,   /* Q^T X and Q S on bf16 MFMA, FP32 accumulation    */
+    ORTH_PREC_FP16 = 2,   /* same with fp16 operands                          */
+    ORTH_PREC_BF16X3 = 3  /* split bf16x3 operands (hi*hi + hi*lo + lo*hi)   */
+};
+
+typedef struct {
+    int info_block;  /* 1-based block whose Cholesky broke down first, 0 = none */
+    int info_pivot;  /* 1-based pivot of that breakdown                         */
+    int hip_error;   /* last hipError_t seen by the library (0 = none)          */
+    int passes;      /* streaming passes over the row slab launched by the call */
+    int syncs;       /* global reductions (the reference's MPI_Allreduce count) */
+} orth_status_t;
+
+/* Status of the last completed call on the calling thread's device;
+ * waits for that call's work to finish.  Returns 0. */
+int orth_last_status(orth_status_t *st);
+
+/* Device form of the five variants: dX (m x n, ld ldX) becomes Q in place,
+ * dR (ld ldR) receives R exactly as in orth.h.  Returns 0, or 2 for an
+ * invalid shape / HIP error. */
+int orth_bcgs_device(int variant, int64_t m, int64_t n, int64_t s, double *dX,
+                     int64_t ldX, double *dR, int64_t ldR, void *stream);
+
+/* Accuracy harness of the reference (SRC/util.c:36-75) on device data:
+ * out[0] = ||Q^T Q - I||_F, out[1] = ||Q R - X||_F, out[2] = ||X||_F.
+ * Synchronous; out is host memory. */
+int orth_accuracy_device(int64_t m, int64_t n, const double *dX, int64_t ldX,
+                         const double *dQ, int64_t ldQ, const double *dR,
+                         int64_t ldR, double *out, void *stream);
+
+/* Block Gram primitive: dC (na x nb, ld ldc, device) = A^T B over m rows
+ * with A = columns [a0, a0+na) and B = columns [b0, b0+nb) of dX; the
+ * reduction order is fixed, so results are bitwise reproducible. */
+int orth_gram_device(int64_t m, const double *dX, int64_t ldX, int64_t a0,
+                     int64_t na, int64_t b0, int64_t nb, double *dC,
+                     int64_t ldc, void *stream);
+
+/* Block update primitive (one row-local pass over the slab):
+ *   X(:, col0:col0+w) <- (X(:, col0:col0+w) - X(:, 0:c) W) T^{-1}
+ * W is c x w (ld ldW, device), T is w x w upper (ld w, device) or NULL.
+ * Requires c <= col0 and w <= 32.  Synchronous. */
+int orth_update_device(int64_t m, double *dX, int64_t ldX, int64_t col0,
+                       int64_t w, int64_t c, const double *dW, int64_t ldW,
+                       const double *dT, void *stream);
+
+/* Kernel classes for orth_kernel_times. */
+enum {
+    ORTH_K_PROJ = 0,    /* projection partials A^T B (MFMA FP64)       */
+    ORTH_K_REDUCE = 1,  /* fixed-order sum of the partials             */
+    ORTH_K_UPDATE = 2,  /* row-local block updates                     */
+    ORTH_K_SMALL = 3,   /* one-workgroup s x s steps                   */
+    ORTH_K_PASS = 4,    /* fused update + projection pass              */
+    ORTH_K_COUNT = 8
+};
+
+/* Enable (1) or disable (0) per-launch event timing on this device and
+ * clear what was recorded.  Events are recorded on the stream of each
+ * launch, around the launch. */
+int orth_kernel_timing(int enable);
+
+/* Total elapsed milliseconds, launch count, algorithmic flops and
+ * algorithmic HBM bytes recorded for one kernel class since the last
+ * orth_kernel_timing call (waits for the device).  flops/bytes may be NULL. */
+int orth_kernel_times(int kind, double *total_ms, int *count, double *flops,
+                      double *bytes);
+
+/* Wait for all work the library enqueued on this device. */
+int orth_sync(void);
+
+/* Release the library's device scratch on this device. */
+int orth_release(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,816 @@
+// Host side of liborth_hip.so: the C-ABI entry points of include/orth.h and
+// include/orth_ext.h, and the per-variant schedules of passes over the row
+// slab.
+//
+// Every MPI_Allreduce of the reference becomes one projection pass whose
+// per-workgroup partials are summed in a fixed order on the device; the s x s
+// work between passes runs in a one-workgroup kernel.  Nothing returns to
+// the host between blocks: a call enqueues its whole schedule on one stream.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <math.h>
+
+#include <algorithm>
+#include <mutex>
+#include <vector>
+
+#include "../../include/orth.h"
+#include "../../include/orth_ext.h"
+#include "orth_kernels.h"
+
+namespace orth {
+namespace {
+
+constexpr int kMaxDevices = 64;
+constexpr int kProjTargetWG = 1024;
+
+struct DevCtx {
+    std::mutex mu;
+    bool init = false;
+    hipEvent_t done = nullptr;  // end of the last enqueued call
+    bool done_valid = false;
+    double *part = nullptr;
+    size_t part_n = 0;
+    double *P[2] = {nullptr, nullptr};
+    size_t P_n = 0;
+    double *stash = nullptr;  // Sig | Om | Cq[3] | Ctmp
+    double *Rd = nullptr;
+    size_t Rd_n = 0;
+    double *Xd = nullptr;
+    size_t Xd_n = 0;
+    int *status_d = nullptr;
+    int *status_h = nullptr;
+    orth_status_t last = {0, 0, 0, 0, 0};
+    int last_passes = 0, last_syncs = 0;
+    // Optional per-launch event timing, by kernel class (ORTH_K_*).
+    bool ktime = false;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> kt[ORTH_K_COUNT];
+    double kt_flops[ORTH_K_COUNT] = {0};  // algorithmic flops of the timed launches
+    double kt_bytes[ORTH_K_COUNT] = {0};  // algorithmic HBM bytes of the timed launches
+};
+
+DevCtx g_ctx[kMaxDevices];
+
+thread_local int tl_hip_error = 0;
+
+bool ok(hipError_t e)
+{
+    if (e != hipSuccess) {
+        tl_hip_error = (int)e;
+        return false;
+    }
+    return true;
+}
+
+bool ensure(double *&p, size_t &cap, size_t n)
+{
+    if (cap >= n && p) return true;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    if (!ok(hipMalloc(&p, n * sizeof(double)))) return false;
+    cap = n;
+    return true;
+}
+
+int current_device()
+{
+    int d = 0;
+    (void)hipGetDevice(&d);
+    return d;
+}
+
+bool ctx_init(DevCtx &c)
+{
+    if (c.init) return true;
+    if (!ok(hipEventCreateWithFlags(&c.done, hipEventDisableTiming))) return false;
+    if (!ok(hipMalloc(&c.stash, 6 * kLdT * kLdT * sizeof(double)))) return false;
+    if (!ok(hipMalloc(&c.status_d, 4 * sizeof(int)))) return false;
+    if (!ok(hipHostMalloc(&c.status_h, 4 * sizeof(int), hipHostMallocDefault))) return false;
+    c.init = true;
+    return true;
+}
+
+int block_template(int64_t s)
+{
+    if (s <= 4) return 4;
+    if (s <= 8) return 8;
+    if (s <= 16) return 16;
+    if (s <= 32) return 32;
+    return 0;
+}
+
+// Records start/stop events around one launch when timing is enabled.
+template <class F>
+bool timed(DevCtx &c, int kind, hipStream_t st, double flops, double bytes, F launch)
+{
+    if (!c.ktime) return ok(launch());
+    c.kt_flops[kind] += flops;
+    c.kt_bytes[kind] += bytes;
+    while (c.ev_pool.size() < c.ev_used + 2) {
+        hipEvent_t e;
+        if (!ok(hipEventCreate(&e))) return false;
+        c.ev_pool.push_back(e);
+    }
+    hipEvent_t a = c.ev_pool[c.ev_used], b = c.ev_pool[c.ev_used + 1];
+    c.ev_used += 2;
+    if (!ok(hipEventRecord(a, st))) return false;
+    if (!ok(launch())) return false;
+    if (!ok(hipEventRecord(b, st))) return false;
+    c.kt[kind].push_back({a, b});
+    return true;
+}
+
+// One orthogonalization call: the schedule of passes for one variant.
+class Run {
+  public:
+    Run(DevCtx &c, int64_t m, int64_t n, int64_t s, double *X, int64_t ldX, hipStream_t st)
+        : c_(c), m_(m), n_(n), s_(s), X_(X), ldX_(ldX), st_(st)
+    {
+        S_ = block_template(s);
+        N_ = 1 + (n - s + s - 1) / s;
+        ldR_ = n;
+        Sig_ = c.stash;
+        Om_ = Sig_ + kLdT * kLdT;
+        Cq_ = Om_ + kLdT * kLdT;
+        Ctmp_ = Cq_ + 3 * kLdT * kLdT;
+    }
+
+    bool prepare()
+    {
+        // Largest projection: (n + 3 s) x (3 s) covers every schedule; the
+        // extra kMaxS columns keep coefficient reads past a ragged block in
+        // bounds.
+        const size_t pmax = (size_t)(n_ + 3 * kMaxS) * (4 * kMaxS) + 64;
+        if (c_.P_n < pmax || !c_.P[0] || !c_.P[1]) {
+            for (int k = 0; k < 2; k++) {
+                if (c_.P[k]) (void)hipFree(c_.P[k]);
+                c_.P[k] = nullptr;
+            }
+            c_.P_n = 0;
+            if (!ok(hipMalloc(&c_.P[0], pmax * sizeof(double)))) return false;
+            if (!ok(hipMalloc(&c_.P[1], pmax * sizeof(double)))) return false;
+            c_.P_n = pmax;
+        }
+        const size_t partmax = (size_t)kProjTargetWG * (n_ + 3 * kMaxS) * (3 * kMaxS);
+        if (!ensure(c_.part, c_.part_n, partmax)) return false;
+        const size_t rmax = (size_t)n_ * (n_ + kMaxS) + 64;
+        if (!ensure(c_.Rd, c_.Rd_n, rmax)) return false;
+        if (!ok(launch_fill(c_.Rd, rmax, st_))) return false;
+        if (!ok(launch_fill(c_.stash, 6 * kLdT * kLdT, st_))) return false;
+        if (!ok(launch_fill_int(c_.status_d, 4, st_))) return false;
+        Rd_ = c_.Rd;
+        return true;
+    }
+
+    int64_t bw(int64_t b) const { return b == 0 ? s_ : std::min<int64_t>(s_, n_ - b * s_); }
+
+    // P <- [X(:, a0:a0+na)]^T [X(:, b0:b0+nb)]; returns the buffer (ld na).
+    double *proj(int64_t a0, int64_t na, int64_t b0, int64_t nb)
+    {
+        ProjPlan pl = plan_proj(m_, na, nb, kProjTargetWG);
+        const size_t need = (size_t)pl.nwg * na * nb;
+        if (!ensure(c_.part, c_.part_n, need)) return nullptr;
+        if (pl.lds_bytes > 64 * 1024) return nullptr;
+        // Algorithmic work: 2 flops per row and output entry; each column of
+        // A union B read once.
+        const int64_t lo = std::max(a0, b0), hi = std::min(a0 + na, b0 + nb);
+        const int64_t ucols = na + nb - std::max<int64_t>(0, hi - lo);
+        if (!timed(c_, ORTH_K_PROJ, st_, 2.0 * m_ * na * nb, 8.0 * m_ * ucols, [&] {
+                return launch_proj(pl, X_, ldX_, m_, a0, na, b0, nb, c_.part, st_);
+            }))
+            return nullptr;
+        pcur_ ^= 1;
+        double *P = c_.P[pcur_];
+        if (!timed(c_, ORTH_K_REDUCE, st_, (double)pl.nwg * na * nb,
+                   8.0 * ((double)pl.nwg + 1) * na * nb,
+                   [&] { return launch_reduce(c_.part, pl.nwg, na, nb, P, na, st_); }))
+            return nullptr;
+        passes_++;
+        syncs_++;
+        return P;
+    }
+
+    bool small(SmallArgs a)
+    {
+        a.m = m_;
+        a.R = Rd_;
+        a.ldR = ldR_;
+        a.Sig = Sig_;
+        a.Om = Om_;
+        a.Cq = Cq_;
+        a.Ctmp = Ctmp_;
+        a.status = c_.status_d;
+        if (!timed(c_, ORTH_K_SMALL, st_, 0.0, 0.0, [&] { return launch_small(a, st_); }))
+            return false;
+        if (debug_) dump(a);
+        return true;
+    }
+
+    static SmallArgs sa(int op, int flags, int64_t blk, int64_t c, int64_t wb, int64_t wn,
+                        double *P, int64_t na)
+    {
+        SmallArgs a;
+        memset(&a, 0, sizeof(a));
+        a.op = op;
+        a.flags = flags;
+        a.blk = (int)blk;
+        a.c = (int)c;
+        a.wb = (int)wb;
+        a.wn = (int)wn;
+        a.P = P;
+        a.na = (int)na;
+        return a;
+    }
+
+    static Upd upd(int64_t col0, int64_t w, int64_t c, const double *W, int64_t ldW,
+                   const double *T)
+    {
+        Upd u;
+        u.col0 = col0;
+        u.w = (int)w;
+        u.c = (int)c;
+        u.W = W;
+        u.ldW = ldW;
+        u.T = T;
+        return u;
+    }
+
+    // Algorithmic work of an update: 2 c w + w^2 flops per row (product and
+    // triangular solve); prefix and target read once, target written once.
+    double upd_flops(const Upd &u) const
+    {
+        return (double)m_ * (2.0 * u.c * u.w + (u.T ? (double)u.w * u.w : 0.0));
+    }
+    bool update(const Upd &u0)
+    {
+        passes_++;
+        return timed(c_, ORTH_K_UPDATE, st_, upd_flops(u0), 8.0 * m_ * (u0.c + 2.0 * u0.w),
+                     [&] { return launch_update(S_, X_, ldX_, m_, &u0, 1, st_); });
+    }
+    bool update2(const Upd &u0, const Upd &u1)
+    {
+        Upd u[2] = {u0, u1};
+        passes_++;
+        const double bytes = 8.0 * m_ * (u1.c + u1.w + u0.w + u1.w);
+        return timed(c_, ORTH_K_UPDATE, st_, upd_flops(u0) + upd_flops(u1), bytes,
+                     [&] { return launch_update(S_, X_, ldX_, m_, u, 2, st_); });
+    }
+
+    // Intra-block QR of columns [col0, col0+w): shifted CholQR3; the factor
+    // R3 R2 R1 goes to Rdst (ld ldRdst).  Replaces Trilinos TSQR
+    // factorExplicit (SRC/orth.c:106-107 and the other TSQR call sites).
+    bool intra_qr(int64_t blk, int64_t col0, int64_t w, double *Rdst, int64_t ldRdst)
+    {
+        const int ops[3] = {OP_CQR1, OP_CQR2, OP_CQR3};
+        for (int k = 0; k < 3; k++) {
+            double *P = proj(col0, w, col0, w);
+            if (!P) return false;
+            SmallArgs a = sa(ops[k], 0, blk, 0, w, 0, P, w);
+            a.Rdst = Rdst;
+            a.ldRdst = ldRdst;
+            if (!small(a)) return false;
+            if (!update(upd(col0, w, 0, nullptr, 0, Cq_ + k * kLdT * kLdT))) return false;
+        }
+        return true;
+    }
+
+    // ---------------- variant schedules ----------------
+
+    // BCGSI+P-1S (SRC/orth.c:48-222).
+    bool p1s()
+    {
+        if (!intra_qr(0, 0, s_, Rd_, ldR_)) return false;  // :105-107
+        if (N_ == 1) return true;
+        const int64_t w1 = bw(1);
+        {
+            // [Q1 X2]^T X2 (:111-114); Sigma, R12 = S (:115-119).
+            double *P = proj(0, s_ + w1, s_, w1);
+            if (!P || !small(sa(OP_OPEN_PYTH, 0, 1, s_, w1, 0, P, s_ + w1))) return false;
+            // X2 = (X2 - Q1 S) Sigma^{-1} (:121-124).
+            if (!update(upd(s_, w1, s_, P, s_ + w1, Sig_))) return false;
+        }
+        int64_t c = s_;
+        for (int64_t b = 1; b < N_; b++) {
+            const int64_t wb = bw(b), wn = (b + 1 < N_) ? bw(b + 1) : 0;
+            const int64_t na = c + wb + wn;
+            // The one sync: [Q Q^_b]^T [Q^_b X_n] and X_n^T X_n (:127-133, :182-197).
+            double *P = proj(0, na, c, wb + wn);
+            if (!P || !small(sa(OP_CLOSE_OPEN, F_PYTH_NEXT, b, c, wb, wn, P, na))) return false;
+            // Q_b = (Q^_b - Q Y) Omega^{-1} (:139-142, :206-209) and the next
+            // block's first pass (:173-176) in one pass.
+            Upd u0 = upd(c, wb, c, P, na, Om_);
+            if (wn > 0) {
+                Upd u1 = upd(c + wb, wn, c + wb, P + wb * na, na, Sig_);
+                if (!update2(u0, u1)) return false;
+            } else if (!update(u0)) {
+                return false;
+            }
+            c += wb;
+        }
+        return true;
+    }
+
+    // BCGSI+A-1S (SRC/orth.c:224-367).
+    bool a1s()
+    {
+        if (!intra_qr(0, 0, s_, Rd_, ldR_)) return false;  // :269-271
+        if (N_ == 1) return true;
+        const int64_t w1 = bw(1);
+        {
+            // S = Q1^T X2 (:275-278); V2 = X2 - Q1 S (:280-281).
+            double *P = proj(0, s_, s_, w1);
+            if (!P || !small(sa(OP_PLAIN_S, 0, 1, s_, w1, 0, P, s_))) return false;
+            if (!update(upd(s_, w1, s_, P, s_, nullptr))) return false;
+        }
+        int64_t c = s_;
+        for (int64_t b = 1; b < N_; b++) {
+            const int64_t wb = bw(b), wn = (b + 1 < N_) ? bw(b + 1) : 0;
+            const int64_t na = c + wb;
+            // [Q V_b]^T [V_b X_n] (:284-287, :330-342).
+            double *P = proj(0, na, c, wb + wn);
+            if (!P || !small(sa(OP_CLOSE_OPEN, F_1S, b, c, wb, wn, P, na))) return false;
+            Upd u0 = upd(c, wb, c, P, na, Om_);
+            if (wn > 0) {
+                Upd u1 = upd(c + wb, wn, c + wb, P + wb * na, na, nullptr);
+                if (!update2(u0, u1)) return false;
+            } else if (!update(u0)) {
+                return false;
+            }
+            c += wb;
+        }
+        return true;
+    }
+
+    // BCGS-PIPI+ (SRC/orth.c:369-462).
+    bool pipi()
+    {
+        if (!intra_qr(0, 0, s_, Rd_, ldR_)) return false;  // :411-413
+        int64_t c = s_;
+        for (int64_t b = 1; b < N_; b++) {
+            const int64_t wb = bw(b), na = c + wb;
+            double *P = proj(0, na, c, wb);  // :421-424
+            if (!P || !small(sa(OP_PIPI1, 0, b, c, wb, 0, P, na))) return false;
+            if (!update(upd(c, wb, c, P, na, Sig_))) return false;  // :431-434
+            P = proj(0, na, c, wb);  // :435-438
+            if (!P || !small(sa(OP_PIPI2, 0, b, c, wb, 0, P, na))) return false;
+            if (!update(upd(c, wb, c, P, na, Om_))) return false;  // :446-449
+            c += wb;
+        }
+        return true;
+    }
+
+    // BCGSI+ (SRC/orth.c:464-548).
+    bool i2()
+    {
+        if (!intra_qr(0, 0, s_, Rd_, ldR_)) return false;  // :503-505
+        int64_t c = s_;
+        for (int64_t b = 1; b < N_; b++) {
+            const int64_t wb = bw(b);
+            double *P = proj(0, c, c, wb);  // S = Q^T X (:512-515)
+            if (!P || !small(sa(OP_PLAIN_S, 0, b, c, wb, 0, P, c))) return false;
+            if (!update(upd(c, wb, c, P, c, nullptr))) return false;  // :518
+            if (!intra_qr(b, c, wb, Sig_, kLdT)) return false;         // :520-523
+            P = proj(0, c, c, wb);  // W = Q^T Q^ (:524-527)
+            if (!P || !small(sa(OP_I2_W, 0, b, c, wb, 0, P, c))) return false;
+            if (!update(upd(c, wb, c, P, c, nullptr))) return false;  // :530
+            if (!intra_qr(b, c, wb, Om_, kLdT)) return false;          // :532-535
+            if (!small(sa(OP_RBB, 0, b, c, wb, 0, nullptr, 1))) return false;  // :537-541
+            c += wb;
+        }
+        return true;
+    }
+
+    // BCGSI+P-2S (SRC/orth.c:550-697).
+    bool p2s()
+    {
+        if (!intra_qr(0, 0, s_, Rd_, ldR_)) return false;  // :599-601
+        if (N_ == 1) return true;
+        const int64_t w1 = bw(1);
+        {
+            // S = Q1^T X2, R12 = S (:605-609); V = X2 - Q1 S (:610-611); TSQR (:612-614).
+            double *P = proj(0, s_, s_, w1);
+            if (!P || !small(sa(OP_PLAIN_S, 0, 1, s_, w1, 0, P, s_))) return false;
+            if (!update(upd(s_, w1, s_, P, s_, nullptr))) return false;
+            if (!intra_qr(1, s_, w1, Sig_, kLdT)) return false;
+        }
+        int64_t c = s_;
+        for (int64_t b = 1; b < N_; b++) {
+            const int64_t wb = bw(b), wn = (b + 1 < N_) ? bw(b + 1) : 0;
+            const int64_t na = c + wb;
+            // [Q Q^_b]^T [Q^_b X_n] (:617-620, :658-675); the unused
+            // X_n^T X_n Gram of :660-662 is not formed.
+            double *P = proj(0, na, c, wb + wn);
+            if (!P || !small(sa(OP_CLOSE_OPEN, 0, b, c, wb, wn, P, na))) return false;
+            Upd u0 = upd(c, wb, c, P, na, Om_);
+            if (wn > 0) {
+                Upd u1 = upd(c + wb, wn, c + wb, P + wb * na, na, nullptr);
+                if (!update2(u0, u1)) return false;
+                if (!intra_qr(b + 1, c + wb, wn, Sig_, kLdT)) return false;  // :649-652
+            } else if (!update(u0)) {
+                return false;
+            }
+            c += wb;
+        }
+        return true;
+    }
+
+    bool run(int variant)
+    {
+        switch (variant) {
+        case ORTH_P1S: return p1s();
+        case ORTH_1S: return a1s();
+        case ORTH_PIPI: return pipi();
+        case ORTH_I2: return i2();
+        case ORTH_P2S: return p2s();
+        default: return false;
+        }
+    }
+
+    // Copy R to the caller: for block column k only rows 0 .. end of block k.
+    bool copy_R_out(double *Rdst, int64_t ldRdst, hipMemcpyKind kind)
+    {
+        int64_t c = 0;
+        for (int64_t b = 0; b < N_; b++) {
+            const int64_t w = bw(b);
+            const int64_t rows = c + w;
+            if (!ok(hipMemcpy2DAsync(Rdst + c * ldRdst, ldRdst * sizeof(double),
+                                     Rd_ + c * ldR_, ldR_ * sizeof(double),
+                                     rows * sizeof(double), w, kind, st_)))
+                return false;
+            c += w;
+        }
+        return true;
+    }
+
+    int passes_ = 0, syncs_ = 0;
+    bool debug_ = getenv("ORTH_DEBUG_DUMP") != nullptr;
+
+    // Debug aid (ORTH_DEBUG_DUMP set): print the reduced projection and the
+    // factors after each small step.
+    void dump(const SmallArgs &a)
+    {
+        (void)hipStreamSynchronize(st_);
+        std::vector<double> h(6 * kLdT * kLdT);
+        (void)hipMemcpy(h.data(), c_.stash, h.size() * sizeof(double), hipMemcpyDeviceToHost);
+        int stat[4];
+        (void)hipMemcpy(stat, c_.status_d, sizeof(stat), hipMemcpyDeviceToHost);
+        fprintf(stderr, "[orth] op=%d blk=%d c=%d wb=%d wn=%d na=%d status=%d/%d\n", a.op,
+                a.blk, a.c, a.wb, a.wn, a.na, stat[0], stat[1]);
+        if (a.P) {
+            std::vector<double> p((size_t)a.na * (a.wb + a.wn));
+            (void)hipMemcpy(p.data(), a.P, p.size() * sizeof(double), hipMemcpyDeviceToHost);
+            fprintf(stderr, "  P[0..3][0..3]:");
+            for (int i = 0; i < 4 && i < a.na; i++)
+                for (int j = 0; j < 4 && j < a.wb; j++) fprintf(stderr, " %.6e", p[i + j * a.na]);
+            fprintf(stderr, "\n");
+        }
+        const char *names[6] = {"Sig", "Om", "Cq0", "Cq1", "Cq2", "Ctmp"};
+        for (int k = 0; k < 6; k++) {
+            fprintf(stderr, "  %s diag:", names[k]);
+            for (int i = 0; i < 4; i++) fprintf(stderr, " %.6e", h[k * kLdT * kLdT + i * (kLdT + 1)]);
+            fprintf(stderr, "\n");
+        }
+    }
+
+  private:
+    DevCtx &c_;
+    int64_t m_, n_, s_;
+    double *X_;
+    int64_t ldX_;
+    hipStream_t st_;
+    int S_ = 16;
+    int64_t N_ = 1;
+    double *Rd_ = nullptr;
+    int64_t ldR_ = 0;
+    double *Sig_, *Om_, *Cq_, *Ctmp_;
+    int pcur_ = 0;
+};
+
+bool is_device_ptr(const void *p)
+{
+    if (!p) return false;
+    hipPointerAttribute_t at;
+    memset(&at, 0, sizeof(at));
+    hipError_t e = hipPointerGetAttributes(&at, p);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return at.type == hipMemoryTypeDevice || at.type == hipMemoryTypeManaged;
+}
+
+bool shape_ok(int64_t m, int64_t n, int64_t s, int64_t ldX, int64_t ldR)
+{
+    return s >= 1 && n >= s && m >= n && ldX >= m && ldR >= n && block_template(s) != 0 &&
+           n <= 4096;
+}
+
+// Enqueue one variant on device data.  Rdev_out receives R (device or host
+// memory, per kind).  Returns 0 or 2.
+int run_device(DevCtx &c, int variant, int64_t m, int64_t n, int64_t s, double *dX,
+               int64_t ldX, double *Rout, int64_t ldR, hipMemcpyKind rkind, hipStream_t st)
+{
+    if (c.done_valid && !ok(hipStreamWaitEvent(st, c.done, 0))) return 2;
+    Run r(c, m, n, s, dX, ldX, st);
+    if (!r.prepare()) return 2;
+    if (!r.run(variant)) return 2;
+    if (!r.copy_R_out(Rout, ldR, rkind)) return 2;
+    if (!ok(hipMemcpyAsync(c.status_h, c.status_d, 4 * sizeof(int), hipMemcpyDeviceToHost, st)))
+        return 2;
+    if (!ok(hipEventRecord(c.done, st))) return 2;
+    c.done_valid = true;
+    c.last_passes = r.passes_;
+    c.last_syncs = r.syncs_;
+    return 0;
+}
+
+void fill_status(DevCtx &c)
+{
+    if (c.done_valid) (void)hipEventSynchronize(c.done);
+    c.last.info_block = c.status_h ? c.status_h[0] : 0;
+    c.last.info_pivot = c.status_h ? c.status_h[1] : 0;
+    c.last.hip_error = tl_hip_error;
+    c.last.passes = c.last_passes;
+    c.last.syncs = c.last_syncs;
+}
+
+// Drop-in entry: host or device X / R, synchronous.
+int dropin(int variant, int m, int n, int s, double *X, int ldX, double *R, int ldR,
+           double *work, int lwork)
+{
+    const int64_t need = 5 * (int64_t)n * s + 9 * (int64_t)s * s +
+                         (int64_t)m * (variant == ORTH_I2 ? s : n);
+    if (work != nullptr && (int64_t)lwork < need) {
+        printf("%% Work space is too small. It should be larger than 6*n*s + 8*s*s.\n");
+        return 1;
+    }
+    if (!X || !R || !shape_ok(m, n, s, ldX, ldR)) return 2;
+    tl_hip_error = 0;
+    const int dev = current_device();
+    if (dev < 0 || dev >= kMaxDevices) return 2;
+    DevCtx &c = g_ctx[dev];
+    std::lock_guard<std::mutex> lk(c.mu);
+    if (!ctx_init(c)) return 2;
+    hipStream_t st = nullptr;
+    const bool xdev = is_device_ptr(X), rdev = is_device_ptr(R);
+    double *dX = X;
+    int64_t ldd = ldX;
+    if (!xdev) {
+        if (!ensure(c.Xd, c.Xd_n, (size_t)m * n)) return 2;
+        dX = c.Xd;
+        ldd = m;
+        if (!ok(hipMemcpy2DAsync(dX, (size_t)ldd * sizeof(double), X, (size_t)ldX * sizeof(double),
+                                 (size_t)m * sizeof(double), n, hipMemcpyHostToDevice, st)))
+            return 2;
+    }
+    int rc = run_device(c, variant, m, n, s, dX, ldd, R, ldR,
+                        rdev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, st);
+    if (rc != 0) return rc;
+    if (!xdev) {
+        if (!ok(hipMemcpy2DAsync(X, (size_t)ldX * sizeof(double), dX, (size_t)ldd * sizeof(double),
+                                 (size_t)m * sizeof(double), n, hipMemcpyDeviceToHost, st)))
+            return 2;
+    }
+    if (!ok(hipStreamSynchronize(st))) return 2;
+    fill_status(c);
+    return 0;
+}
+
+}  // namespace
+}  // namespace orth
+
+using namespace orth;
+
+extern "C" {
+
+int bcgsi2P1s(int m, int n, int s, double *Xsub, int ldXsub, double *R, int ldR, double *work,
+              int lwork)
+{
+    return dropin(ORTH_P1S, m, n, s, Xsub, ldXsub, R, ldR, work, lwork);
+}
+
+int bcgsi21s(int m, int n, int s, double *Xsub, int ldXsub, double *R, int ldR, double *work,
+             int lwork)
+{
+    return dropin(ORTH_1S, m, n, s, Xsub, ldXsub, R, ldR, work, lwork);
+}
+
+int bcgspipi2(int m, int n, int s, double *Xsub, int ldXsub, double *R, int ldR, double *work,
+              int lwork)
+{
+    return dropin(ORTH_PIPI, m, n, s, Xsub, ldXsub, R, ldR, work, lwork);
+}
+
+int bcgsi2(int m, int n, int s, double *Xsub, int ldXsub, double *R, int ldR, double *work,
+           int lwork)
+{
+    return dropin(ORTH_I2, m, n, s, Xsub, ldXsub, R, ldR, work, lwork);
+}
+
+int bcgsi2P2s(int m, int n, int s, double *Xsub, int ldXsub, double *R, int ldR, double *work,
+              int lwork)
+{
+    return dropin(ORTH_P2S, m, n, s, Xsub, ldXsub, R, ldR, work, lwork);
+}
+
+int orth_last_status(orth_status_t *st)
+{
+    if (!st) return 2;
+    const int dev = current_device();
+    DevCtx &c = g_ctx[dev];
+    std::lock_guard<std::mutex> lk(c.mu);
+    fill_status(c);
+    *st = c.last;
+    return 0;
+}
+
+int orth_bcgs_device(int variant, int64_t m, int64_t n, int64_t s, double *dX, int64_t ldX,
+                     double *dR, int64_t ldR, void *stream)
+{
+    if (!dX || !dR || !shape_ok(m, n, s, ldX, ldR)) return 2;
+    tl_hip_error = 0;
+    const int dev = current_device();
+    DevCtx &c = g_ctx[dev];
+    std::lock_guard<std::mutex> lk(c.mu);
+    if (!ctx_init(c)) return 2;
+    return run_device(c, variant, m, n, s, dX, ldX, dR, ldR, hipMemcpyDeviceToDevice,
+                      (hipStream_t)stream);
+}
+
+int orth_gram_device(int64_t m, const double *dX, int64_t ldX, int64_t a0, int64_t na,
+                     int64_t b0, int64_t nb, double *dC, int64_t ldc, void *stream)
+{
+    if (!dX || !dC || m < 1 || na < 1 || nb < 1 || ldc < na) return 2;
+    tl_hip_error = 0;
+    const int dev = current_device();
+    DevCtx &c = g_ctx[dev];
+    std::lock_guard<std::mutex> lk(c.mu);
+    if (!ctx_init(c)) return 2;
+    hipStream_t st = (hipStream_t)stream;
+    if (c.done_valid && !ok(hipStreamWaitEvent(st, c.done, 0))) return 2;
+    ProjPlan pl = plan_proj(m, na, nb, kProjTargetWG);
+    if (pl.lds_bytes > 64 * 1024) return 2;
+    if (!ensure(c.part, c.part_n, (size_t)pl.nwg * na * nb)) return 2;
+    if (!ok(launch_proj(pl, dX, ldX, m, a0, na, b0, nb, c.part, st))) return 2;
+    if (!ok(launch_reduce(c.part, pl.nwg, na, nb, dC, ldc, st))) return 2;
+    if (!ok(hipEventRecord(c.done, st))) return 2;
+    c.done_valid = true;
+    return 0;
+}
+
+int orth_accuracy_device(int64_t m, int64_t n, const double *dX, int64_t ldX, const double *dQ,
+                         int64_t ldQ, const double *dR, int64_t ldR, double *out, void *stream)
+{
+    if (!out || m < 1 || n < 1) return 2;
+    hipStream_t st = (hipStream_t)stream;
+    double *G = nullptr;
+    if (!ok(hipMalloc(&G, (size_t)n * n * sizeof(double)))) return 2;
+    int rc = orth_gram_device(m, dQ, ldQ, 0, n, 0, n, G, n, stream);
+    if (rc != 0) {
+        (void)hipFree(G);
+        return rc;
+    }
+    std::vector<double> Gh((size_t)n * n);
+    const int nwg = resid_wg_count(m);
+    double *part2 = nullptr;
+    if (!ok(hipMalloc(&part2, (size_t)2 * nwg * sizeof(double)))) {
+        (void)hipFree(G);
+        return 2;
+    }
+    std::vector<double> ph((size_t)2 * nwg);
+    bool good = ok(launch_resid(m, n, dX, ldX, dQ, ldQ, dR, ldR, part2, st)) &&
+                ok(hipMemcpyAsync(Gh.data(), G, Gh.size() * sizeof(double),
+                                  hipMemcpyDeviceToHost, st)) &&
+                ok(hipMemcpyAsync(ph.data(), part2, ph.size() * sizeof(double),
+                                  hipMemcpyDeviceToHost, st)) &&
+                ok(hipStreamSynchronize(st));
+    (void)hipFree(G);
+    (void)hipFree(part2);
+    if (!good) return 2;
+    double loo = 0.0;
+    for (int64_t j = 0; j < n; j++)
+        for (int64_t i = 0; i < n; i++) {
+            const double v = Gh[i + j * n] - (i == j ? 1.0 : 0.0);
+            loo += v * v;
+        }
+    double sr = 0.0, sx = 0.0;
+    for (int k = 0; k < nwg; k++) {
+        sr += ph[2 * k];
+        sx += ph[2 * k + 1];
+    }
+    out[0] = sqrt(loo);
+    out[1] = sqrt(sr);
+    out[2] = sqrt(sx);
+    return 0;
+}
+
+int orth_kernel_timing(int enable)
+{
+    const int dev = current_device();
+    DevCtx &c = g_ctx[dev];
+    std::lock_guard<std::mutex> lk(c.mu);
+    if (c.done_valid) (void)hipEventSynchronize(c.done);
+    c.ktime = enable != 0;
+    c.ev_used = 0;
+    for (auto &v : c.kt) v.clear();
+    for (int k = 0; k < ORTH_K_COUNT; k++) c.kt_flops[k] = c.kt_bytes[k] = 0.0;
+    return 0;
+}
+
+int orth_kernel_times(int kind, double *total_ms, int *count, double *flops, double *bytes)
+{
+    if (kind < 0 || kind >= ORTH_K_COUNT || !total_ms || !count) return 2;
+    const int dev = current_device();
+    DevCtx &c = g_ctx[dev];
+    std::lock_guard<std::mutex> lk(c.mu);
+    if (c.done_valid && !ok(hipEventSynchronize(c.done))) return 2;
+    double t = 0.0;
+    for (auto &pr : c.kt[kind]) {
+        float ms = 0.0f;
+        if (!ok(hipEventElapsedTime(&ms, pr.first, pr.second))) return 2;
+        t += ms;
+    }
+    *total_ms = t;
+    *count = (int)c.kt[kind].size();
+    if (flops) *flops = c.kt_flops[kind];
+    if (bytes) *bytes = c.kt_bytes[kind];
+    return 0;
+}
+
+int orth_update_device(int64_t m, double *dX, int64_t ldX, int64_t col0, int64_t w,
+                       int64_t c, const double *dW, int64_t ldW, const double *dT,
+                       void *stream)
+{
+    const int S = block_template(w);
+    if (!dX || m < 1 || w < 1 || S == 0 || c < 0 || col0 < c || (c > 0 && !dW) ||
+        (c > 0 && ldW < c))
+        return 2;
+    tl_hip_error = 0;
+    const int dev = current_device();
+    DevCtx &ctx = g_ctx[dev];
+    std::lock_guard<std::mutex> lk(ctx.mu);
+    if (!ctx_init(ctx)) return 2;
+    hipStream_t st = (hipStream_t)stream;
+    if (ctx.done_valid && !ok(hipStreamWaitEvent(st, ctx.done, 0))) return 2;
+    // Coefficients are staged with S zero-padded columns and the factor with
+    // leading dimension kLdT, the layout the update kernel reads.
+    double *Wp = nullptr, *Tp = nullptr;
+    if (c > 0) {
+        if (!ok(hipMalloc(&Wp, (size_t)c * S * sizeof(double)))) return 2;
+        if (!ok(launch_fill(Wp, (size_t)c * S, st)) ||
+            !ok(hipMemcpy2DAsync(Wp, c * sizeof(double), dW, ldW * sizeof(double),
+                                 c * sizeof(double), w, hipMemcpyDeviceToDevice, st)))
+            return 2;
+    }
+    if (dT) {
+        if (!ok(hipMalloc(&Tp, (size_t)kLdT * kLdT * sizeof(double)))) return 2;
+        if (!ok(launch_fill(Tp, (size_t)kLdT * kLdT, st)) ||
+            !ok(hipMemcpy2DAsync(Tp, kLdT * sizeof(double), dT, w * sizeof(double),
+                                 w * sizeof(double), w, hipMemcpyDeviceToDevice, st)))
+            return 2;
+    }
+    Upd u;
+    u.col0 = col0;
+    u.w = (int)w;
+    u.c = (int)c;
+    u.W = Wp;
+    u.ldW = c;
+    u.T = Tp;
+    bool good = ok(launch_update(S, dX, ldX, m, &u, 1, st)) && ok(hipStreamSynchronize(st));
+    if (Wp) (void)hipFree(Wp);
+    if (Tp) (void)hipFree(Tp);
+    return good ? 0 : 2;
+}
+
+int orth_sync(void)
+{
+    const int dev = current_device();
+    DevCtx &c = g_ctx[dev];
+    std::lock_guard<std::mutex> lk(c.mu);
+    if (c.done_valid && !ok(hipEventSynchronize(c.done))) return 2;
+    return 0;
+}
+
+int orth_release(void)
+{
+    const int dev = current_device();
+    DevCtx &c = g_ctx[dev];
+    std::lock_guard<std::mutex> lk(c.mu);
+    if (c.done_valid) (void)hipEventSynchronize(c.done);
+    double **bufs[] = {&c.part, &c.P[0], &c.P[1], &c.Rd, &c.Xd};
+    for (double **b : bufs) {
+        if (*b) (void)hipFree(*b);
+        *b = nullptr;
+    }
+    c.part_n = c.P_n = c.Rd_n = c.Xd_n = 0;
+    return 0;
+}
+
+}  // extern "C"

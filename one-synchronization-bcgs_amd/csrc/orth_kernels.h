@@ -1,0 +1,100 @@
+// Device kernels of liborth_hip.so and their launch wrappers.
+//
+// Data layout: X is the caller's m x n column-major row slab (leading
+// dimension ldX) and is updated in place, one block column at a time.  The
+// small matrices (projections, Cholesky factors, R) live in library scratch.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace orth {
+
+constexpr int kMaxS = 32;  // widest block the kernels support
+constexpr int kLdT = 32;   // leading dimension of every s x s factor
+
+// One row-local block update of a streaming pass:
+//   X(:, col0:col0+w) <- (X(:, col0:col0+w) - X(:, 0:c) * W) * T^{-1}
+// W is c x w (ld ldW; columns w..S-1 must be readable, their values unused),
+// T is w x w upper with ld kLdT, or nullptr for no normalization.
+struct Upd {
+    int64_t col0;
+    int w;
+    int c;
+    const double *W;
+    int64_t ldW;
+    const double *T;
+};
+
+// Small-matrix steps run by one workgroup between passes.
+enum SmallOp {
+    OP_OPEN_PYTH = 1,   // P1s prologue: Sigma = chol(G - S^T S), R(0:c,b) = S
+    OP_CLOSE_OPEN = 2,  // finish block b (Omega, R) and open block b+1
+    OP_PIPI1 = 3,       // PIPI first pass: Sigma, R(0:c,b) = S
+    OP_PIPI2 = 4,       // PIPI second pass: Omega, R(0:c,b) += Y Sigma, R_bb
+    OP_PLAIN_S = 5,     // R(0:c,b) = S (unnormalized first pass)
+    OP_I2_W = 6,        // BCGSI+: R(0:c,b) += W Sigma
+    OP_RBB = 7,         // R_bb = Omega Sigma
+    OP_CQR1 = 8,        // shifted CholQR step 1
+    OP_CQR2 = 9,        // CholQR step 2
+    OP_CQR3 = 10        // CholQR step 3, writes R3 R2 R1
+};
+enum SmallFlag {
+    F_PYTH_NEXT = 1,  // next block's Sigma from the Pythagorean Gram (P1s)
+    F_1S = 2          // R(0:c,b) = S + Y and R_bb = Omega (BCGSI+A-1S)
+};
+
+struct SmallArgs {
+    int op;
+    int flags;
+    int blk;  // 0-based block index b
+    int c;    // columns final before block b (= first column of block b)
+    int wb;   // width of block b
+    int wn;   // width of block b+1 (0 if none)
+    int na;   // leading dimension of P
+    int64_t m;
+    double *P;
+    double *R;
+    int64_t ldR;
+    double *Sig, *Om, *Cq, *Ctmp;  // s x s stashes, ld kLdT (Cq holds 3)
+    double *Rdst;
+    int64_t ldRdst;
+    int *status;  // [0] = first breakdown block (1-based), [1] = pivot
+};
+
+hipError_t launch_update(int S, double *X, int64_t ldX, int64_t m,
+                         const Upd *u, int nupd, hipStream_t st);
+
+// Projection partials: part[x][i + j*na] = sum over the rows of workgroup x
+// of A(:, i) * B(:, j), A = columns [a0, a0+na), B = columns [b0, b0+nb).
+struct ProjPlan {
+    int Ma, Nb;        // 16-wide tiles of A and B
+    int ga, gb;        // tiles per group along A and B
+    int ngy;           // number of tile groups (gridDim.y)
+    int nwg;           // row chunks (gridDim.x)
+    int64_t rows_per;  // rows per chunk (multiple of 16)
+    size_t lds_bytes;
+};
+ProjPlan plan_proj(int64_t m, int64_t na, int64_t nb, int target_wg);
+hipError_t launch_proj(const ProjPlan &pl, const double *X, int64_t ldX,
+                       int64_t m, int64_t a0, int64_t na, int64_t b0,
+                       int64_t nb, double *part, hipStream_t st);
+// out[i + j*ldo] = sum_x part[x][i + j*na], fixed order.
+hipError_t launch_reduce(const double *part, int nwg, int64_t na, int64_t nb,
+                         double *out, int64_t ldo, hipStream_t st);
+
+hipError_t launch_small(const SmallArgs &a, hipStream_t st);
+
+// Zero n entries with a kernel (no memset nodes, so the sequence can be
+// captured in a graph).
+hipError_t launch_fill(double *p, size_t n, hipStream_t st);
+hipError_t launch_fill_int(int *p, size_t n, hipStream_t st);
+
+// Residual harness: per-workgroup partial sums of ||Q R - X||^2 and ||X||^2
+// into part2[2*x], part2[2*x+1]; returns the number of workgroups used.
+int resid_wg_count(int64_t m);
+hipError_t launch_resid(int64_t m, int64_t n, const double *X, int64_t ldX,
+                        const double *Q, int64_t ldQ, const double *R,
+                        int64_t ldR, double *part2, hipStream_t st);
+
+}  // namespace orth

@@ -1,0 +1,558 @@
+// Device kernels of liborth_hip.so (gfx950).
+//
+//  k_update  row-local block updates, one row per lane, coefficients read as
+//            wave-uniform (scalar) loads
+//  k_proj    A^T B over a row slab on v_mfma_f64_16x16x4f64, tiles staged
+//            through LDS; one partial per workgroup
+//  k_reduce  fixed-order sum of the partials (the in-kernel replacement of
+//            the reference's MPI_Allreduce)
+//  k_small   the s x s / c x s steps between passes, one workgroup, FP64
+//  k_resid   ||Q R - X||_F and ||X||_F partial sums (accuracy harness)
+#include "orth_kernels.h"
+
+#include <math.h>
+
+namespace orth {
+
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+// ---------------------------------------------------------------------------
+// Row-local updates
+// ---------------------------------------------------------------------------
+
+template <int S>
+__device__ __forceinline__ void apply_upd(double *__restrict__ xr, int64_t ldX,
+                                          const Upd &u)
+{
+    double acc[S];
+#pragma unroll
+    for (int i = 0; i < S; i++) acc[i] = (i < u.w) ? xr[(u.col0 + i) * ldX] : 0.0;
+    const int c = u.c;
+    int j = 0;
+    for (; j + 4 <= c; j += 4) {
+        const double q0 = xr[(int64_t)j * ldX];
+        const double q1 = xr[(int64_t)(j + 1) * ldX];
+        const double q2 = xr[(int64_t)(j + 2) * ldX];
+        const double q3 = xr[(int64_t)(j + 3) * ldX];
+        const double *w = u.W + j;
+#pragma unroll
+        for (int i = 0; i < S; i++) {
+            const double *wi = w + i * u.ldW;
+            double a = acc[i];
+            a = fma(-q0, wi[0], a);
+            a = fma(-q1, wi[1], a);
+            a = fma(-q2, wi[2], a);
+            a = fma(-q3, wi[3], a);
+            acc[i] = a;
+        }
+    }
+    for (; j < c; j++) {
+        const double q = xr[(int64_t)j * ldX];
+#pragma unroll
+        for (int i = 0; i < S; i++) acc[i] = fma(-q, u.W[j + i * u.ldW], acc[i]);
+    }
+    if (u.T) {
+        // Row-wise forward substitution with the upper factor (dtrsm R,U,N,N).
+#pragma unroll
+        for (int i = 0; i < S; i++) {
+            if (i < u.w) {
+                double v = acc[i];
+#pragma unroll
+                for (int l = 0; l < i; l++) v = fma(-acc[l], u.T[l + i * kLdT], v);
+                acc[i] = v * (1.0 / u.T[i + i * kLdT]);
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < S; i++)
+        if (i < u.w) xr[(u.col0 + i) * ldX] = acc[i];
+}
+
+template <int S>
+__global__ __launch_bounds__(256) void k_update(double *X, int64_t ldX, int64_t m,
+                                                Upd u0, Upd u1, int nupd)
+{
+    const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (r >= m) return;
+    double *xr = X + r;
+    apply_upd<S>(xr, ldX, u0);
+    if (nupd > 1) apply_upd<S>(xr, ldX, u1);
+}
+
+hipError_t launch_update(int S, double *X, int64_t ldX, int64_t m,
+                         const Upd *u, int nupd, hipStream_t st)
+{
+    if (m <= 0 || nupd <= 0) return hipSuccess;
+    dim3 grid((unsigned)((m + 255) / 256)), block(256);
+    Upd u1 = nupd > 1 ? u[1] : u[0];
+    switch (S) {
+    case 4: k_update<4><<<grid, block, 0, st>>>(X, ldX, m, u[0], u1, nupd); break;
+    case 8: k_update<8><<<grid, block, 0, st>>>(X, ldX, m, u[0], u1, nupd); break;
+    case 16: k_update<16><<<grid, block, 0, st>>>(X, ldX, m, u[0], u1, nupd); break;
+    case 32: k_update<32><<<grid, block, 0, st>>>(X, ldX, m, u[0], u1, nupd); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Projection A^T B on FP64 MFMA
+// ---------------------------------------------------------------------------
+
+constexpr int kRT = 16;       // rows per LDS tile
+constexpr int kLdsLd = kRT + 1;  // padded column stride in LDS (doubles)
+constexpr int kMaxTilesPerWave = 12;
+constexpr int kMaxLdsCols = 30 * 16;
+
+ProjPlan plan_proj(int64_t m, int64_t na, int64_t nb, int target_wg)
+{
+    ProjPlan p;
+    p.Ma = (int)((na + 15) / 16);
+    p.Nb = (int)((nb + 15) / 16);
+    p.gb = p.Nb <= 4 ? p.Nb : (p.Nb < 8 ? p.Nb : 8);
+    int ga = 4 * kMaxTilesPerWave / p.gb;
+    if (ga > kMaxLdsCols / 16 - p.gb) ga = kMaxLdsCols / 16 - p.gb;
+    if (ga > p.Ma) ga = p.Ma;
+    if (ga < 1) ga = 1;
+    p.ga = ga;
+    p.ngy = ((p.Ma + p.ga - 1) / p.ga) * ((p.Nb + p.gb - 1) / p.gb);
+    int64_t chunks = (m + kRT - 1) / kRT;
+    int64_t want = target_wg / p.ngy;
+    if (want < 1) want = 1;
+    if (want > chunks) want = chunks;
+    if (want < 1) want = 1;
+    int64_t rows = (m + want - 1) / want;
+    rows = ((rows + kRT - 1) / kRT) * kRT;
+    if (rows < kRT) rows = kRT;
+    p.rows_per = rows;
+    p.nwg = (int)((m + rows - 1) / rows);
+    if (p.nwg < 1) p.nwg = 1;
+    p.lds_bytes = (size_t)(p.ga + p.gb) * 16 * kLdsLd * sizeof(double);
+    return p;
+}
+
+__global__ __launch_bounds__(256) void k_proj(const double *__restrict__ X, int64_t ldX,
+                                              int64_t m, int64_t a0, int na, int64_t b0,
+                                              int nb, int Ma, int Nb, int ga, int gb,
+                                              int64_t rows_per, double *__restrict__ part)
+{
+    extern __shared__ double lds[];
+    const int ngb = (Nb + gb - 1) / gb;
+    const int gy = blockIdx.y;
+    const int ta0 = (gy / ngb) * ga, tb0 = (gy % ngb) * gb;
+    const int nta = min(ga, Ma - ta0), ntb = min(gb, Nb - tb0);
+    const int colsA = nta * 16, ncol = colsA + ntb * 16;
+    const int ntiles = nta * ntb;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int li = lane & 15, lk = lane >> 4;
+
+    dbl4 acc[kMaxTilesPerWave];
+#pragma unroll
+    for (int t = 0; t < kMaxTilesPerWave; t++) acc[t] = dbl4{0.0, 0.0, 0.0, 0.0};
+
+    const int64_t rb = (int64_t)blockIdx.x * rows_per;
+    const int64_t re = min(m, rb + rows_per);
+    for (int64_t r0 = rb; r0 < re; r0 += kRT) {
+        __syncthreads();
+        for (int e = threadIdx.x; e < ncol * kRT; e += 256) {
+            const int cl = e / kRT, rr = e - cl * kRT;
+            const int64_t row = r0 + rr;
+            int64_t gcol;
+            bool ok;
+            if (cl < colsA) {
+                const int ac = ta0 * 16 + cl;
+                ok = ac < na;
+                gcol = a0 + ac;
+            } else {
+                const int bc = tb0 * 16 + (cl - colsA);
+                ok = bc < nb;
+                gcol = b0 + bc;
+            }
+            double v = 0.0;
+            if (ok && row < re) v = X[row + gcol * ldX];
+            lds[cl * kLdsLd + rr] = v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < kRT / 4; ks++) {
+            const int rr = ks * 4 + lk;
+#pragma unroll
+            for (int t = 0; t < kMaxTilesPerWave; t++) {
+                const int tile = wave + 4 * t;
+                if (tile < ntiles) {
+                    const int ta = tile / ntb, tb = tile - ta * ntb;
+                    const double a = lds[(ta * 16 + li) * kLdsLd + rr];
+                    const double b = lds[(colsA + tb * 16 + li) * kLdsLd + rr];
+                    acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[t], 0, 0, 0);
+                }
+            }
+        }
+    }
+    // D layout of v_mfma_f64_16x16x4f64: lane (li, lk), value v holds row
+    // lk + 4 v, column li of the tile.
+    double *out = part + (int64_t)blockIdx.x * na * nb;
+#pragma unroll
+    for (int t = 0; t < kMaxTilesPerWave; t++) {
+        const int tile = wave + 4 * t;
+        if (tile < ntiles) {
+            const int ta = tile / ntb, tb = tile - ta * ntb;
+            const int j = (tb0 + tb) * 16 + li;
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int i = (ta0 + ta) * 16 + lk + 4 * v;
+                if (i < na && j < nb) out[i + (int64_t)j * na] = acc[t][v];
+            }
+        }
+    }
+}
+
+hipError_t launch_proj(const ProjPlan &pl, const double *X, int64_t ldX, int64_t m,
+                       int64_t a0, int64_t na, int64_t b0, int64_t nb, double *part,
+                       hipStream_t st)
+{
+    dim3 grid(pl.nwg, pl.ngy), block(256);
+    k_proj<<<grid, block, pl.lds_bytes, st>>>(X, ldX, m, a0, (int)na, b0, (int)nb, pl.Ma,
+                                              pl.Nb, pl.ga, pl.gb, pl.rows_per, part);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_reduce(const double *__restrict__ part, int nwg,
+                                                int64_t na, int64_t E,
+                                                double *__restrict__ out, int64_t ldo)
+{
+    __shared__ double red[8][32];
+    const int el = threadIdx.x & 31, g = threadIdx.x >> 5;
+    const int64_t e = (int64_t)blockIdx.x * 32 + el;
+    double s = 0.0;
+    if (e < E)
+        for (int w = g; w < nwg; w += 8) s += part[(int64_t)w * E + e];
+    red[g][el] = s;
+    __syncthreads();
+    if (g == 0 && e < E) {
+        double t = 0.0;
+#pragma unroll
+        for (int k = 0; k < 8; k++) t += red[k][el];
+        const int64_t i = e % na, j = e / na;
+        out[i + j * ldo] = t;
+    }
+}
+
+hipError_t launch_reduce(const double *part, int nwg, int64_t na, int64_t nb, double *out,
+                         int64_t ldo, hipStream_t st)
+{
+    const int64_t E = na * nb;
+    if (E <= 0) return hipSuccess;
+    k_reduce<<<dim3((unsigned)((E + 31) / 32)), dim3(256), 0, st>>>(part, nwg, na, E, out, ldo);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Small-matrix steps (one workgroup of 256 threads, FP64)
+// ---------------------------------------------------------------------------
+
+// C(ma x nb) = beta*C + alpha * A^T B, A: k x ma, B: k x nb.
+__device__ void d_tn(int k, int ma, const double *A, int64_t lda, int nb, const double *B,
+                     int64_t ldb, double *C, int64_t ldc, double alpha, double beta)
+{
+    for (int e = threadIdx.x; e < ma * nb; e += blockDim.x) {
+        const int i = e % ma, j = e / ma;
+        double s = 0.0;
+        for (int l = 0; l < k; l++) s = fma(A[l + i * lda], B[l + j * ldb], s);
+        C[i + j * ldc] = beta * C[i + j * ldc] + alpha * s;
+    }
+    __syncthreads();
+}
+
+// C(ma x nb) = beta*C + alpha * A B, A: ma x k, B: k x nb.
+__device__ void d_nn(int ma, int k, const double *A, int64_t lda, int nb, const double *B,
+                     int64_t ldb, double *C, int64_t ldc, double alpha, double beta)
+{
+    for (int e = threadIdx.x; e < ma * nb; e += blockDim.x) {
+        const int i = e % ma, j = e / ma;
+        double s = 0.0;
+        for (int l = 0; l < k; l++) s = fma(A[i + l * lda], B[l + j * ldb], s);
+        C[i + j * ldc] = beta * C[i + j * ldc] + alpha * s;
+    }
+    __syncthreads();
+}
+
+__device__ void d_copy(int rows, int cols, const double *A, int64_t lda, double *B, int64_t ldb)
+{
+    for (int e = threadIdx.x; e < rows * cols; e += blockDim.x) {
+        const int i = e % rows, j = e / rows;
+        B[i + j * ldb] = A[i + j * lda];
+    }
+    __syncthreads();
+}
+
+// T (w x w, ld kLdT) = chol(upper(G) + shift*I), strict lower zeroed.
+// Returns 0 or the 1-based first non-positive pivot.
+__device__ int d_chol(int w, const double *G, int64_t ldg, double shift, double *T)
+{
+    __shared__ double sh[kMaxS][kMaxS + 1];
+    __shared__ int info_s;
+    for (int e = threadIdx.x; e < w * w; e += blockDim.x) {
+        const int i = e % w, j = e / w;
+        sh[i][j] = (i <= j) ? G[i + j * ldg] + (i == j ? shift : 0.0) : 0.0;
+    }
+    if (threadIdx.x == 0) info_s = 0;
+    __syncthreads();
+    for (int j = 0; j < w; j++) {
+        if (threadIdx.x == 0) {
+            double d = sh[j][j];
+            for (int k = 0; k < j; k++) d = fma(-sh[k][j], sh[k][j], d);
+            if (!(d > 0.0) && info_s == 0) info_s = j + 1;
+            sh[j][j] = sqrt(d);
+        }
+        __syncthreads();
+        const double rinv = 1.0 / sh[j][j];
+        for (int i = j + 1 + threadIdx.x; i < w; i += blockDim.x) {
+            double v = sh[j][i];
+            for (int k = 0; k < j; k++) v = fma(-sh[k][j], sh[k][i], v);
+            sh[j][i] = v * rinv;
+        }
+        __syncthreads();
+    }
+    for (int e = threadIdx.x; e < w * w; e += blockDim.x) {
+        const int i = e % w, j = e / w;
+        T[i + j * kLdT] = (i <= j) ? sh[i][j] : 0.0;
+    }
+    const int info = info_s;
+    __syncthreads();
+    return info;
+}
+
+// B (w x k) = U^{-T} B with U upper (ld kLdT): dtrsm_('L','U','T','N').
+__device__ void d_trsm_lut(int w, int k, const double *U, double *B, int64_t ldb)
+{
+    for (int j = threadIdx.x; j < k; j += blockDim.x) {
+        for (int i = 0; i < w; i++) {
+            double t = B[i + j * ldb];
+            for (int l = 0; l < i; l++) t = fma(-U[l + i * kLdT], B[l + j * ldb], t);
+            B[i + j * ldb] = t / U[i + i * kLdT];
+        }
+    }
+    __syncthreads();
+}
+
+// C = A B for upper-triangular A, B (ld kLdT); C strict lower zeroed.
+__device__ void d_trmm(int w, const double *A, const double *B, double *C, int64_t ldc)
+{
+    for (int e = threadIdx.x; e < w * w; e += blockDim.x) {
+        const int i = e % w, j = e / w;
+        double s = 0.0;
+        for (int l = i; l <= j; l++) s = fma(A[i + l * kLdT], B[l + j * kLdT], s);
+        C[i + j * ldc] = (i <= j) ? s : 0.0;
+    }
+    __syncthreads();
+}
+
+__device__ void note_info(int *status, int blk, int info)
+{
+    if (threadIdx.x == 0 && info != 0 && status[0] == 0) {
+        status[0] = blk + 1;
+        status[1] = info;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_small(SmallArgs a)
+{
+    double *P = a.P;
+    const int c = a.c, wb = a.wb, wn = a.wn, na = a.na;
+    double *Rb = a.R + (int64_t)c * a.ldR;         // R(:, block b)
+    double *Rbb = Rb + c;                          // R(b, b)
+    switch (a.op) {
+    case OP_OPEN_PYTH: {
+        // G -= S^T S; R(0:c, b) = S; Sigma = chol(G)  (SRC/orth.c:115-119).
+        d_tn(c, wb, P, na, wb, P, na, P + c, na, -1.0, 1.0);
+        d_copy(c, wb, P, na, Rb, a.ldR);
+        note_info(a.status, a.blk, d_chol(wb, P + c, na, 0.0, a.Sig));
+        break;
+    }
+    case OP_CLOSE_OPEN: {
+        // Omega = chol(Q^T Q^ - Y^T Y) (SRC/orth.c:134-137, :200-204).
+        d_tn(c, wb, P, na, wb, P, na, P + c, na, -1.0, 1.0);
+        note_info(a.status, a.blk, d_chol(wb, P + c, na, 0.0, a.Om));
+        if (a.flags & F_1S) {
+            // R(0:c, b) = S + Y; R_bb = Omega (SRC/orth.c:356-360).
+            for (int e = threadIdx.x; e < c * wb; e += blockDim.x) {
+                const int i = e % c, j = e / c;
+                Rb[i + j * a.ldR] += P[i + j * na];
+            }
+            __syncthreads();
+            d_copy(wb, wb, a.Om, kLdT, Rbb, a.ldR);
+        } else {
+            // R(0:c, b) = S + Y Sigma; R_bb = Omega Sigma (SRC/orth.c:211-215).
+            d_nn(c, wb, P, na, wb, a.Sig, kLdT, Rb, a.ldR, 1.0, 1.0);
+            d_trmm(wb, a.Om, a.Sig, Rbb, a.ldR);
+        }
+        if (wn > 0) {
+            // Lag correction of the next block's projection (SRC/orth.c:155-159).
+            double *Pn = P + (int64_t)wb * na;
+            d_tn(c, wb, P, na, wn, Pn, na, Pn + c, na, -1.0, 1.0);
+            d_trsm_lut(wb, wn, a.Om, Pn + c, na);
+            d_copy(c + wb, wn, Pn, na, Rb + (int64_t)wb * a.ldR, a.ldR);
+            if (a.flags & F_PYTH_NEXT) {
+                // Sigma_n = chol(X_n^T X_n - S^T S) (SRC/orth.c:160-171).
+                double *H = Pn + c + wb;
+                d_tn(c + wb, wn, Pn, na, wn, Pn, na, H, na, -1.0, 1.0);
+                note_info(a.status, a.blk + 1, d_chol(wn, H, na, 0.0, a.Sig));
+            }
+        }
+        break;
+    }
+    case OP_PIPI1: {
+        // Sigma = chol(X^T X - S^T S); R(0:c, b) = S (SRC/orth.c:425-429).
+        d_tn(c, wb, P, na, wb, P, na, P + c, na, -1.0, 1.0);
+        note_info(a.status, a.blk, d_chol(wb, P + c, na, 0.0, a.Sig));
+        d_copy(c, wb, P, na, Rb, a.ldR);
+        break;
+    }
+    case OP_PIPI2: {
+        // Omega; R(0:c, b) += Y Sigma; R_bb = Omega Sigma (SRC/orth.c:440-455).
+        d_tn(c, wb, P, na, wb, P, na, P + c, na, -1.0, 1.0);
+        note_info(a.status, a.blk, d_chol(wb, P + c, na, 0.0, a.Om));
+        d_nn(c, wb, P, na, wb, a.Sig, kLdT, Rb, a.ldR, 1.0, 1.0);
+        d_trmm(wb, a.Om, a.Sig, Rbb, a.ldR);
+        break;
+    }
+    case OP_PLAIN_S: {
+        d_copy(c, wb, P, na, Rb, a.ldR);
+        break;
+    }
+    case OP_I2_W: {
+        d_nn(c, wb, P, na, wb, a.Sig, kLdT, Rb, a.ldR, 1.0, 1.0);
+        break;
+    }
+    case OP_RBB: {
+        d_trmm(wb, a.Om, a.Sig, Rbb, a.ldR);
+        break;
+    }
+    case OP_CQR1: {
+        // Shift of Fukaya et al. (2020): 11 (m w + w (w+1)) u ||X||_F^2.
+        __shared__ double tr;
+        if (threadIdx.x == 0) {
+            double t = 0.0;
+            for (int i = 0; i < wb; i++) t += P[i + i * na];
+            tr = t;
+        }
+        __syncthreads();
+        const double u = 1.1102230246251565e-16;
+        const double shift = 11.0 * ((double)a.m * wb + (double)wb * (wb + 1)) * u * tr;
+        note_info(a.status, a.blk, d_chol(wb, P, na, shift, a.Cq));
+        break;
+    }
+    case OP_CQR2: {
+        note_info(a.status, a.blk, d_chol(wb, P, na, 0.0, a.Cq + kLdT * kLdT));
+        break;
+    }
+    case OP_CQR3: {
+        double *C2 = a.Cq + 2 * kLdT * kLdT;
+        note_info(a.status, a.blk, d_chol(wb, P, na, 0.0, C2));
+        d_trmm(wb, C2, a.Cq + kLdT * kLdT, a.Ctmp, kLdT);
+        d_trmm(wb, a.Ctmp, a.Cq, a.Rdst, a.ldRdst);
+        break;
+    }
+    default: break;
+    }
+}
+
+hipError_t launch_small(const SmallArgs &a, hipStream_t st)
+{
+    k_small<<<dim3(1), dim3(256), 0, st>>>(a);
+    return hipGetLastError();
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_fill(T *p, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
+        p[i] = T(0);
+}
+
+hipError_t launch_fill(double *p, size_t n, hipStream_t st)
+{
+    if (n == 0) return hipSuccess;
+    size_t g = (n + 255) / 256;
+    if (g > 4096) g = 4096;
+    k_fill<double><<<dim3((unsigned)g), dim3(256), 0, st>>>(p, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_int(int *p, size_t n, hipStream_t st)
+{
+    if (n == 0) return hipSuccess;
+    size_t g = (n + 255) / 256;
+    if (g > 4096) g = 4096;
+    k_fill<int><<<dim3((unsigned)g), dim3(256), 0, st>>>(p, n);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Residual of the accuracy harness
+// ---------------------------------------------------------------------------
+
+constexpr int kResidWG = 1024;
+
+int resid_wg_count(int64_t m)
+{
+    int64_t g = (m + 255) / 256;
+    return (int)(g < kResidWG ? g : kResidWG);
+}
+
+__global__ __launch_bounds__(256) void k_resid(int64_t m, int n, const double *__restrict__ X,
+                                               int64_t ldX, const double *__restrict__ Q,
+                                               int64_t ldQ, const double *__restrict__ R,
+                                               int64_t ldR, double *part2)
+{
+    __shared__ double red[2][256];
+    double sr = 0.0, sx = 0.0;
+    for (int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x; r < m;
+         r += (int64_t)gridDim.x * 256) {
+        for (int j0 = 0; j0 < n; j0 += 16) {
+            double t[16];
+#pragma unroll
+            for (int jj = 0; jj < 16; jj++) {
+                const int j = j0 + jj;
+                const double x = j < n ? X[r + j * ldX] : 0.0;
+                sx = fma(x, x, sx);
+                t[jj] = -x;
+            }
+            for (int i = 0; i < n; i++) {
+                const double q = Q[r + i * ldQ];
+#pragma unroll
+                for (int jj = 0; jj < 16; jj++) {
+                    const int j = j0 + jj;
+                    if (j < n) t[jj] = fma(q, R[i + j * ldR], t[jj]);
+                }
+            }
+#pragma unroll
+            for (int jj = 0; jj < 16; jj++) sr = fma(t[jj], t[jj], sr);
+        }
+    }
+    red[0][threadIdx.x] = sr;
+    red[1][threadIdx.x] = sx;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + h];
+            red[1][threadIdx.x] += red[1][threadIdx.x + h];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part2[2 * blockIdx.x] = red[0][0];
+        part2[2 * blockIdx.x + 1] = red[1][0];
+    }
+}
+
+hipError_t launch_resid(int64_t m, int64_t n, const double *X, int64_t ldX, const double *Q,
+                        int64_t ldQ, const double *R, int64_t ldR, double *part2,
+                        hipStream_t st)
+{
+    k_resid<<<dim3(resid_wg_count(m)), dim3(256), 0, st>>>(m, (int)n, X, ldX, Q, ldQ, R, ldR,
+                                                           part2);
+    return hipGetLastError();
+}
+
+}  // namespace orth

@@ -48,6 +48,39 @@ typedef int64_t idx;
 
 static int g_threads = 1;
 
+/* Mistake planted on purpose by the tests (0 = none), to show that their
+ * checks reject a wrong restatement:
+ *   1 drop the Y*Sigma term of R(0:c, b)
+ *   2 add Y^T Y in the Pythagorean Gram of the second pass
+ *   3 skip Omega^{-T} in the lag correction of the next block's projection
+ *   4 leave the last prefix column out of the second-pass update
+ *   5 multiply R_bb = Sigma Omega instead of Omega Sigma */
+static int g_plant = 0;
+void oracle_plant(int mistake) { g_plant = mistake; }
+
+/* Relative perturbation of every first-pass projection S (0 = none).  The
+ * second pass must remove it; with it the reorthogonalization terms are no
+ * longer rounding-level, so the tests can pin them.  It also models a
+ * first pass computed in reduced precision. */
+static double g_noise = 0.0;
+void oracle_set_first_pass_noise(double eta) { g_noise = eta; }
+
+static void perturb(idx rows, idx cols, double *S, idx ld, idx blk)
+{
+    if (g_noise == 0.0) return;
+    for (idx j = 0; j < cols; j++)
+        for (idx i = 0; i < rows; i++) {
+            uint64_t z = (uint64_t)(i + 1) * 0x9E3779B97F4A7C15ull ^
+                         (uint64_t)(j + 7) * 0xBF58476D1CE4E5B9ull ^
+                         (uint64_t)(blk + 13) * 0x94D049BB133111EBull;
+            z ^= z >> 31;
+            z *= 0xD6E8FEB86648D5A5ull;
+            z ^= z >> 29;
+            const double r = (double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+            S[i + j * ld] *= 1.0 + g_noise * r;
+        }
+}
+
 void oracle_set_threads(int nthreads) { g_threads = nthreads < 1 ? 1 : nthreads; }
 int oracle_get_threads(void) { return g_threads; }
 
@@ -399,18 +432,19 @@ static void second_pass(const ctx *c, idx b, idx cc, idx wb, idx wn,
     /* Omega = chol(Q^_b^T Q^_b - Y^T Y) (:134-137, :200-204, :288-291,
      * :345-349, :621-624, :676-680). */
     double *G = P + cc; /* rows cc.., cols 0..wb */
-    tn_small(cc, wb, P, na, wb, P, na, G, na, -1.0, 1.0);
+    tn_small(cc, wb, P, na, wb, P, na, G, na, g_plant == 2 ? 1.0 : -1.0, 1.0);
     note(c, b, chol_into(wb, G, na, Om));
     /* Q_b = (Q^_b - Q Y) Omega^{-1} (:139-142, :206-209, :293-296, :351-354,
      * :625-628, :681-684). */
-    oracle_update(c->m, wb, colp(c, cc), c->ldX, cc, colp(c, 0), c->ldX, P,
-                  na, Om, wb);
+    oracle_update(c->m, wb, colp(c, cc), c->ldX, g_plant == 4 ? cc - 1 : cc,
+                  colp(c, 0), c->ldX, P, na, Om, wb);
     if (wn > 0) {
         /* S-correction (:155-159, :310-314, :641-645): rows cc..cc+wb of
          * [Q Q^_b]^T X_next become Omega^{-T}(Q^_b^T X_n - Y^T Q^T X_n). */
         double *Pn = P + wb * na; /* column wb of P */
         tn_small(cc, wb, P, na, wn, Pn, na, Pn + cc, na, -1.0, 1.0);
-        trsm_lut(wb, wn, Om, wb, Pn + cc, na);
+        if (g_plant != 3) trsm_lut(wb, wn, Om, wb, Pn + cc, na);
+        perturb(cc + wb, wn, Pn, na, b + 1);
         copy_block(cc + wb, wn, Pn, na, Rp(c, 0, cc + wb), c->ldR);
     }
 }
@@ -430,6 +464,7 @@ static void run_p1s(const ctx *c)
         /* [Q1 X2]^T X2 (:111-114), Pythagorean Sigma (:115-119). */
         idx na = s + w1;
         oracle_gram(m, na, colp(c, 0), c->ldX, w1, colp(c, s), c->ldX, P, na);
+        perturb(s, w1, P, na, 1);
         tn_small(s, w1, P, na, w1, P, na, P + s, na, -1.0, 1.0);
         copy_block(s, w1, P, na, Rp(c, 0, s), c->ldR); /* :117 */
         note(c, 1, chol_into(w1, P + s, na, Sig));
@@ -444,8 +479,12 @@ static void run_p1s(const ctx *c)
         oracle_gram(m, na, colp(c, 0), c->ldX, wb + wn, colp(c, cc), c->ldX, P, na);
         second_pass(c, b, cc, wb, wn, P, na, Om);
         /* R(0:c, b) = S + Y Sigma; R_bb = Omega Sigma (:144-147, :211-215). */
-        nn_small(cc, wb, P, na, wb, Sig, wb, Rp(c, 0, cc), c->ldR, 1.0, 1.0);
-        trmm_uu(wb, Om, wb, Sig, wb, Rp(c, cc, cc), c->ldR);
+        if (g_plant != 1)
+            nn_small(cc, wb, P, na, wb, Sig, wb, Rp(c, 0, cc), c->ldR, 1.0, 1.0);
+        if (g_plant == 5)
+            trmm_uu(wb, Sig, wb, Om, wb, Rp(c, cc, cc), c->ldR);
+        else
+            trmm_uu(wb, Om, wb, Sig, wb, Rp(c, cc, cc), c->ldR);
         if (wn > 0) {
             /* Sigma_n = chol(X_n^T X_n - S^T S) (:160-171). */
             double *Sn = Rp(c, 0, cc + wb);
@@ -475,6 +514,7 @@ static void run_1s(const ctx *c)
     idx w1 = bw(c, 1);
     /* S = Q1^T X2 (:275-278); X2 -= Q1 S without normalization (:280-281). */
     oracle_gram(m, s, colp(c, 0), c->ldX, w1, colp(c, s), c->ldX, Rp(c, 0, s), c->ldR);
+    perturb(s, w1, Rp(c, 0, s), c->ldR, 1);
     oracle_update(m, w1, colp(c, s), c->ldX, s, colp(c, 0), c->ldX, Rp(c, 0, s), c->ldR, NULL, 0);
     idx cc = s;
     for (idx b = 1; b < N; b++) {
@@ -513,6 +553,7 @@ static void run_pipi(const ctx *c)
         const idx wb = bw(c, b), na = cc + wb;
         /* [Q X]^T X (:421-424); Sigma = chol(X^T X - S^T S) (:425-429). */
         oracle_gram(m, na, colp(c, 0), c->ldX, wb, colp(c, cc), c->ldX, P, na);
+        perturb(cc, wb, P, na, b);
         tn_small(cc, wb, P, na, wb, P, na, P + cc, na, -1.0, 1.0);
         note(c, b, chol_into(wb, P + cc, na, Sig));
         /* X = (X - Q S) Sigma^{-1} (:431-434). */
@@ -549,6 +590,7 @@ static void run_i2(const ctx *c)
         const idx wb = bw(c, b);
         /* S = Q^T X (:512-515); X -= Q S (:518); TSQR (:520-523). */
         oracle_gram(m, cc, colp(c, 0), c->ldX, wb, colp(c, cc), c->ldX, S, cc);
+        perturb(cc, wb, S, cc, b);
         oracle_update(m, wb, colp(c, cc), c->ldX, cc, colp(c, 0), c->ldX, S, cc, NULL, 0);
         oracle_houseqr(m, wb, colp(c, cc), c->ldX, Sig, wb);
         /* W = Q^T Q^ (:524-527); Q^ -= Q W (:530); TSQR (:532-535). */
@@ -580,6 +622,7 @@ static void run_p2s(const ctx *c)
     /* S = Q1^T X2 (:605-608), R12 = S (:609), X2 -= Q1 S (:610-611),
      * TSQR of X2 (:612-614). */
     oracle_gram(m, s, colp(c, 0), c->ldX, w1, colp(c, s), c->ldX, Rp(c, 0, s), c->ldR);
+    perturb(s, w1, Rp(c, 0, s), c->ldR, 1);
     oracle_update(m, w1, colp(c, s), c->ldX, s, colp(c, 0), c->ldX, Rp(c, 0, s), c->ldR, NULL, 0);
     oracle_houseqr(m, w1, colp(c, s), c->ldX, Sig, w1);
     idx cc = s;

@@ -68,6 +68,12 @@ void oracle_accuracy(int64_t m, int64_t n, const double *X, int64_t ldX,
                      const double *Q, int64_t ldQ, const double *R,
                      int64_t ldR, double *out);
 
+/* Plant a numbered mistake (tests only, 0 = none); see bcgs_oracle.c. */
+void oracle_plant(int mistake);
+
+/* Relative perturbation of every first-pass projection (tests only). */
+void oracle_set_first_pass_noise(double eta);
+
 /* Primitives, exposed so the tests can pin them against numpy. */
 void oracle_gram(int64_t m, int64_t na, const double *A, int64_t lda,
                  int64_t nb, const double *B, int64_t ldb, double *C,

@@ -62,6 +62,10 @@ def lib():
         L.oracle_update.restype = None
         L.oracle_update.argtypes = [_i64, _i64, _dp, _i64, _i64, _dp, _i64,
                                     _dp, _i64, _dp, _i64]
+        L.oracle_set_first_pass_noise.restype = None
+        L.oracle_set_first_pass_noise.argtypes = [ctypes.c_double]
+        L.oracle_plant.restype = None
+        L.oracle_plant.argtypes = [ctypes.c_int]
         L.oracle_set_threads.restype = None
         L.oracle_set_threads.argtypes = [ctypes.c_int]
         L.oracle_get_threads.restype = ctypes.c_int
@@ -74,6 +78,16 @@ def ptr(a):
         return None
     assert a.dtype == np.float64
     return a.ctypes.data_as(_dp)
+
+
+def plant(mistake):
+    """Plant a numbered mistake in the oracle (0 removes it)."""
+    lib().oracle_plant(int(mistake))
+
+
+def set_first_pass_noise(eta):
+    """Perturb every first-pass projection by a relative eta (0 removes it)."""
+    lib().oracle_set_first_pass_noise(float(eta))
 
 
 def set_threads(n):

@@ -1,0 +1,77 @@
+"""C-ABI checks that need no GPU: liborth_hip.so loads, exports every
+function declared in include/*.h, and rejects bad arguments before touching
+the device (workspace rule of SRC/orth.c:82-86, invalid shapes)."""
+import ctypes
+import os
+import re
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import bcgs_hip as B
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    for h in ("orth.h", "orth_ext.h"):
+        text = open(os.path.join(ROOT, "include", h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^\s*(?:int|void|double)\s+\**(\w+)\s*\(", text, re.M):
+            names.add(m.group(1))
+    return sorted(names)
+
+
+def test_headers_declare_the_reference_symbols():
+    names = declared_functions()
+    for ref in ("bcgsi2P1s", "bcgsi21s", "bcgspipi2", "bcgsi2", "bcgsi2P2s"):
+        assert ref in names
+
+
+def test_library_exports_every_declared_symbol():
+    assert os.path.exists(B.LIB_PATH), "build liborth_hip.so first"
+    out = subprocess.run(["nm", "-D", "--defined-only", B.LIB_PATH],
+                         capture_output=True, text=True, check=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if " T " in line}
+    missing = [n for n in declared_functions() if n not in exported]
+    assert not missing, missing
+    L = B.lib()
+    for n in declared_functions():
+        assert getattr(L, n) is not None
+
+
+def test_reference_signatures_are_c_linkage():
+    L = ctypes.CDLL(B.LIB_PATH)
+    for n in B.VARIANTS:
+        assert hasattr(L, n)
+
+
+@pytest.mark.parametrize("variant", list(B.VARIANTS))
+def test_small_workspace_returns_1_without_device(variant):
+    m, n, s = 200, 20, 5
+    X = np.random.default_rng(0).standard_normal((m, n))
+    need = 5 * n * s + 9 * s * s + m * (s if variant == "bcgsi2" else n)
+    rc, Q, R = B.dropin(variant, X, s, lwork=need - 1, work=True)
+    assert rc == 1
+    assert np.array_equal(Q, X) and not R.any()
+
+
+@pytest.mark.parametrize("shape", [(10, 8, 0), (10, 8, 9), (5, 8, 2), (100, 80, 40)])
+def test_invalid_shape_returns_2_without_device(shape):
+    m, n, s = shape
+    X = np.zeros((m, n))
+    rc, Q, R = B.dropin("bcgsi2P1s", X, s)
+    assert rc == 2
+
+
+def test_no_cpu_fallback_when_library_missing(tmp_path):
+    code = ("import sys; sys.path.insert(0, %r); import bcgs_hip as B; "
+            "B.LIB_PATH = %r\n"
+            "try:\n    B.lib()\nexcept RuntimeError as e:\n    print('raised', e)\n"
+            % (os.path.dirname(B.__file__), str(tmp_path / "nope.so")))
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True,
+                         text=True).stdout
+    assert out.startswith("raised")

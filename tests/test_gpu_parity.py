@@ -1,0 +1,204 @@
+"""GPU parity: liborth_hip.so (through the C ABI) against the CPU oracle on
+the same seeded inputs, plus size-independent properties at larger sizes.
+
+Tolerances (FP64 everywhere): R and Q agree with the oracle to
+1e-11 * kappa-scaled max-norm for well-conditioned inputs; LOO and the
+relative residual agree within 10x (or both sit below 1e-14 / 1e-15);
+Cholesky breakdown is reported for the same block as the oracle.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import bcgs_hip as B
+import inputs as I
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLDEN = os.path.join(HERE, "golden")
+VARIANTS = list(B.VARIANTS)
+
+try:
+    import torch
+except Exception:  # pragma: no cover - torch is part of the image
+    torch = None
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def within10(a, b, floor):
+    if a <= floor and b <= floor:
+        return True
+    return a <= 10 * max(b, floor) and b <= 10 * max(a, floor)
+
+
+def check_against_oracle(variant, X, s, rtol=1e-11):
+    say("case", variant, X.shape, "s=%d" % s)
+    Qo, Ro, info_o = O.bcgs(variant, X, s)
+    Qg, Rg, st = B.bcgs(variant, X, s)
+    assert st["hip_error"] == 0
+    assert (st["info_block"], st["info_pivot"]) == info_o
+    if info_o != (0, 0):
+        return Qg, Rg, st
+    scale = np.abs(Ro).max()
+    assert np.abs(Rg - Ro).max() <= rtol * scale
+    assert np.abs(Qg - Qo).max() <= rtol * 10
+    assert np.all(np.tril(Rg, -1) == 0)
+    assert within10(I.loo(Qg), I.loo(Qo), 1e-14)
+    assert within10(I.rel_residual(X, Qg, Rg), I.rel_residual(X, Qo, Ro), 1e-15)
+    return Qg, Rg, st
+
+
+@pytest.mark.parametrize("variant", VARIANTS)
+@pytest.mark.parametrize("n", [35, 36, 37, 38, 39])
+def test_reference_driver_input(variant, n):
+    g = np.load(os.path.join(GOLDEN, "test_correct.npz"))
+    X = np.asfortranarray(g["X"][:, :n])
+    Qg, Rg, _ = check_against_oracle(variant, X, 5)
+    Rref = g["R%d" % n]
+    assert np.abs(Rg - Rref).max() <= 1e-11 * np.abs(Rref).max()
+
+
+@pytest.mark.parametrize("variant", VARIANTS)
+@pytest.mark.parametrize("shape", [(4096, 64, 16), (4096, 48, 16), (4096, 40, 16),
+                                   (4096, 32, 16), (4096, 16, 16), (3000, 70, 8),
+                                   (3000, 13, 4), (2000, 96, 32), (5000, 30, 5),
+                                   (777, 21, 3)])
+def test_shapes(variant, shape):
+    m, n, s = shape
+    X = I.svd_matrix(m, n, 1e3, seed=21)
+    check_against_oracle(variant, X, s)
+
+
+@pytest.mark.parametrize("variant", VARIANTS)
+@pytest.mark.parametrize("kappa", [1e2, 1e6, 1e10])
+def test_kappa_sweep(variant, kappa):
+    """Same LOO/residual regime as the oracle; breakdown on the same block."""
+    X = I.svd_matrix(20000, 128, kappa, seed=22)
+    say("case kappa", variant, kappa)
+    Qo, Ro, info_o = O.bcgs(variant, X, 16)
+    Qg, Rg, st = B.bcgs(variant, X, 16)
+    if info_o != (0, 0):
+        assert st["info_block"] > 0
+        return
+    assert st["info_block"] == 0
+    assert within10(I.loo(Qg), I.loo(Qo), 1e-14)
+    assert within10(I.rel_residual(X, Qg, Rg), I.rel_residual(X, Qo, Ro), 1e-15)
+
+
+@pytest.mark.parametrize("variant", ["bcgsi2P1s", "bcgsi2", "bcgsi2P2s"])
+def test_glued_blocks(variant):
+    X = I.glued_matrix(8192, 64, 8, 1e-13)
+    say("case glued", variant)
+    _, _, info_o = O.bcgs(variant, X, 8)
+    Qg, Rg, st = B.bcgs(variant, X, 8)
+    assert st["info_block"] == info_o[0]
+    if info_o == (0, 0):
+        assert I.loo(Qg) < 1e-13
+
+
+def test_deterministic_bits():
+    X = I.svd_matrix(30000, 64, 1e4, seed=23)
+    say("case determinism")
+    a = B.bcgs("bcgsi2P1s", X, 16)
+    b = B.bcgs("bcgsi2P1s", X, 16)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_workspace_and_return_codes(variant):
+    m, n, s = 600, 20, 5
+    X = I.svd_matrix(m, n, 10.0, seed=24)
+    need = 5 * n * s + 9 * s * s + m * (s if variant == "bcgsi2" else n)
+    say("case workspace", variant)
+    rc, Q, R = B.dropin(variant, X, s, lwork=need - 1, work=True)
+    assert rc == 1
+    rc, Q, R = B.dropin(variant, X, s, lwork=need, work=True)
+    assert rc == 0 and I.loo(Q) < 1e-13
+
+
+def test_sync_counts():
+    """One reduction per block for the one-sync variants (plus the three of
+    the block-1 intra QR), two for PIPI+."""
+    X = I.svd_matrix(4096, 64, 1e2, seed=25)
+    N = 4
+    expect = {"bcgsi2P1s": 3 + N, "bcgsi21s": 3 + N, "bcgspipi2": 3 + 2 * (N - 1),
+              "bcgsi2": 3 + (N - 1) * (2 + 6), "bcgsi2P2s": 3 + 1 + 3 + (N - 1) + 3 * (N - 2)}
+    for v, k in expect.items():
+        say("case syncs", v)
+        _, _, st = B.bcgs(v, X, 16)
+        assert st["syncs"] == k, (v, st)
+
+
+@pytest.mark.skipif(torch is None, reason="torch missing")
+def test_device_pointers_match_host_path():
+    m, n, s = 10000, 48, 16
+    X = I.svd_matrix(m, n, 1e5, seed=26)
+    say("case device pointers")
+    Qh, Rh, _ = B.bcgs("bcgsi2P1s", X, s)
+    Xd = torch.from_numpy(np.ascontiguousarray(X.T)).cuda()
+    Rd = torch.zeros(n * n, dtype=torch.float64, device="cuda")
+    B.bcgs_device("bcgsi2P1s", m, n, s, Xd.data_ptr(), m, Rd.data_ptr(), n)
+    B.sync()
+    torch.cuda.synchronize()
+    Qd = Xd.cpu().numpy().T
+    R = Rd.cpu().numpy().reshape((n, n), order="F")
+    assert np.array_equal(Qd, Qh) and np.array_equal(R, Rh)
+
+
+@pytest.mark.skipif(torch is None, reason="torch missing")
+def test_gram_and_accuracy_device():
+    m, n = 50000, 40
+    rng = np.random.default_rng(27)
+    X = rng.standard_normal((m, n))
+    say("case gram/accuracy")
+    Xd = torch.from_numpy(np.ascontiguousarray(X.T)).cuda()
+    C = torch.zeros(n * 24, dtype=torch.float64, device="cuda")
+    B.gram_device(m, Xd.data_ptr(), m, 3, 24, 10, 17, C.data_ptr(), 24)
+    B.sync()
+    Ch = C.cpu().numpy().reshape((24, n), order="F")[:, :17]
+    ref = X[:, 3:27].T @ X[:, 10:27]
+    assert np.abs(Ch - ref).max() <= 1e-12 * np.abs(ref).max()
+    Q, R, _ = O.bcgs("bcgsi2", X, 8)
+    Qn = np.asfortranarray(Q + 1e-7 * rng.standard_normal(Q.shape))
+    Qd = torch.from_numpy(np.ascontiguousarray(Qn.T)).cuda()
+    Rd = torch.from_numpy(np.ascontiguousarray(R.T)).cuda()
+    got = B.accuracy_device(m, n, Xd.data_ptr(), m, Qd.data_ptr(), m, Rd.data_ptr(), n)
+    ref = O.accuracy(X, Qn, R)
+    for g_, r_ in zip(got, ref):
+        assert abs(g_ - r_) <= 1e-8 * r_
+
+
+@pytest.mark.skipif(torch is None, reason="torch missing")
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_large_slab_properties(variant):
+    """m = 2^22, n = 256, s = 16 (one eighth of the benchmark slab): LOO and
+    residual at the oracle's level (same kappa, smaller m: both are
+    size-independent for these variants)."""
+    m, n, s = 1 << 22, 256, 16
+    say("case large", variant)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(28)
+    G = torch.randn((n, m), generator=g, dtype=torch.float64, device="cuda")
+    for _ in range(2):  # CholQR2 for an orthonormal U (rows of G^T)
+        L = torch.linalg.cholesky(G @ G.T)
+        G = torch.linalg.inv(L) @ G
+    V, _ = torch.linalg.qr(torch.randn((n, n), generator=g, dtype=torch.float64, device="cuda"))
+    sig = torch.logspace(0, -6, n, dtype=torch.float64, device="cuda")
+    X = ((V * sig) @ G).contiguous()  # rows of X^T: column-major m x n
+    X0 = X.clone()
+    Rd = torch.zeros(n * n, dtype=torch.float64, device="cuda")
+    B.bcgs_device(variant, m, n, s, X.data_ptr(), m, Rd.data_ptr(), n)
+    B.sync()
+    loo, res, xn = B.accuracy_device(m, n, X0.data_ptr(), m, X.data_ptr(), m, Rd.data_ptr(), n)
+    st = B.status()
+    assert st["info_block"] == 0
+    assert loo < (1e-12 if variant == "bcgsi21s" else 1e-13)
+    assert res / xn < 1e-15
