@@ -15,6 +15,7 @@
 #include <math.h>
 
 #include <algorithm>
+#include <climits>
 #include <mutex>
 #include <vector>
 
@@ -38,6 +39,11 @@ struct DevCtx {
     double *P[2] = {nullptr, nullptr};
     size_t P_n = 0;
     double *stash = nullptr;  // Sig | Om | Cq[3] | Ctmp
+    double *Tbig = nullptr;   // (n + kMaxS) x kMaxS scratch of the small steps
+    size_t Tbig_n = 0;
+    double *Cbuf = nullptr;   // coefficients of the next fused pass
+    size_t Cbuf_n = 0;
+    int ncu = 0;
     double *Rd = nullptr;
     size_t Rd_n = 0;
     double *Xd = nullptr;
@@ -93,6 +99,10 @@ bool ctx_init(DevCtx &c)
     if (!ok(hipMalloc(&c.stash, 6 * kLdT * kLdT * sizeof(double)))) return false;
     if (!ok(hipMalloc(&c.status_d, 4 * sizeof(int)))) return false;
     if (!ok(hipHostMalloc(&c.status_h, 4 * sizeof(int), hipHostMallocDefault))) return false;
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    if (!ok(hipDeviceGetAttribute(&c.ncu, hipDeviceAttributeMultiprocessorCount, dev)))
+        return false;
     c.init = true;
     return true;
 }
@@ -135,6 +145,7 @@ class Run {
     {
         S_ = block_template(s);
         N_ = 1 + (n - s + s - 1) / s;
+        n1_ = std::max(16, 2 * S_);
         ldR_ = n;
         Sig_ = c.stash;
         Om_ = Sig_ + kLdT * kLdT;
@@ -162,6 +173,8 @@ class Run {
         if (!ensure(c_.part, c_.part_n, partmax)) return false;
         const size_t rmax = (size_t)n_ * (n_ + kMaxS) + 64;
         if (!ensure(c_.Rd, c_.Rd_n, rmax)) return false;
+        if (!ensure(c_.Tbig, c_.Tbig_n, (size_t)(n_ + kMaxS) * kMaxS)) return false;
+        if (!ensure(c_.Cbuf, c_.Cbuf_n, (size_t)(n_ + 2 * kMaxS) * 2 * kMaxS)) return false;
         if (!ok(launch_fill(c_.Rd, rmax, st_))) return false;
         if (!ok(launch_fill(c_.stash, 6 * kLdT * kLdT, st_))) return false;
         if (!ok(launch_fill_int(c_.status_d, 4, st_))) return false;
@@ -207,6 +220,12 @@ class Run {
         a.Cq = Cq_;
         a.Ctmp = Ctmp_;
         a.status = c_.status_d;
+        a.Tbig = c_.Tbig;
+        if (fused_) {
+            a.Cout = c_.Cbuf;
+            a.n1 = n1_;
+            a.S = S_;
+        }
         if (!timed(c_, ORTH_K_SMALL, st_, 0.0, 0.0, [&] { return launch_small(a, st_); }))
             return false;
         if (debug_) dump(a);
@@ -421,8 +440,269 @@ class Run {
         return true;
     }
 
+    // ---------------- fused schedules ----------------
+    //
+    // Each small step leaves a pending update (coefficients in Cbuf, target
+    // groups); the next streaming pass applies it and accumulates the next
+    // projection in the same sweep, so every block costs one pass per sync.
+
+    struct Pending {
+        int k1 = 0;
+        int ngrp = 0;
+        int seq = 0;
+        PassGroup g[2];
+    } pend_;
+
+    void pend_group(int64_t t0, int64_t w, int zoff, int64_t k, const double *T)
+    {
+        PassGroup &g = pend_.g[pend_.ngrp++];
+        g.t0 = (int)t0;
+        g.w = (int)w;
+        g.zoff = zoff;
+        g.k = (int)k;
+        g.T = T;
+    }
+
+    // One fused pass: the pending update, then P <- A^T B (na == 0: none).
+    double *pass(int64_t a0, int64_t na, int64_t b0, int64_t nb)
+    {
+        // The tile starts at the first column any part of the pass touches
+        // (column 0 whenever a prefix product is applied).
+        int64_t cb = pend_.k1 > 0 ? 0 : INT64_MAX;
+        for (int g = 0; g < pend_.ngrp; g++) cb = std::min<int64_t>(cb, pend_.g[g].t0);
+        if (na > 0) cb = std::min(cb, std::min(a0, b0));
+        if (cb == INT64_MAX) cb = 0;
+        PassArgs pa;
+        memset(&pa, 0, sizeof(pa));
+        pa.X = X_ + cb * ldX_;
+        pa.ldX = ldX_;
+        pa.m = m_;
+        pa.k1 = pend_.k1;
+        pa.n1 = n1_;
+        pa.C = c_.Cbuf;
+        pa.ngrp = pend_.ngrp;
+        pa.seq = pend_.seq;
+        for (int g = 0; g < pend_.ngrp; g++) {
+            pa.g[g] = pend_.g[g];
+            pa.g[g].t0 -= (int)cb;
+        }
+        int64_t ncols = pend_.k1;
+        for (int g = 0; g < pend_.ngrp; g++)
+            ncols = std::max<int64_t>(ncols, pa.g[g].t0 + pa.g[g].w);
+        if (na > 0) ncols = std::max(ncols, std::max(a0 + na, b0 + nb) - cb);
+        int64_t wl = ncols;
+        if (na > 0) {
+            wl = std::max(wl, a0 - cb + ((na + 15) / 16) * 16);
+            wl = std::max(wl, b0 - cb + ((nb + 15) / 16) * 16);
+        }
+        wl = std::max<int64_t>(wl, (pend_.k1 + 3) & ~3);
+        if (S_ == 16 && na > 0) {
+            // The s = 16 kernel gives each wave ceil(Ma / 4) projection tiles;
+            // the padding columns are zero.
+            const int64_t mt = ((na + 15) / 16 + 3) / 4;
+            wl = std::max(wl, a0 - cb + 64 * mt);
+        }
+        wl = (wl + 15) / 16 * 16;
+        pa.ncols = (int)ncols;
+        pa.wl = (int)wl;
+        pa.a0 = (int)(a0 - cb);
+        pa.na = (int)na;
+        pa.b0 = (int)(b0 - cb);
+        pa.nb = (int)nb;
+        pa.part = c_.part;
+        int nwg = std::max(1, c_.ncu);
+        int64_t rows = (m_ + nwg - 1) / nwg;
+        rows = (rows + kPassRows - 1) / kPassRows * kPassRows;
+        pa.rows_per = rows;
+        nwg = (int)((m_ + rows - 1) / rows);
+        if (na > 0 && !ensure(c_.part, c_.part_n, (size_t)nwg * na * nb)) return nullptr;
+        double flops = 2.0 * m_ * na * nb, wcols = 0.0;
+        for (int g = 0; g < pend_.ngrp; g++) {
+            const double kk = pend_.seq ? pend_.g[g].k : pend_.k1;
+            flops += (double)m_ * pend_.g[g].w * (2.0 * kk + (pend_.g[g].T ? pend_.g[g].w : 0));
+            wcols += pend_.g[g].w;
+        }
+        if (!timed(c_, ORTH_K_PASS, st_, flops, 8.0 * m_ * (ncols + wcols),
+                   [&] { return launch_pass(S_, pa, nwg, st_); }))
+            return nullptr;
+        pend_ = Pending();
+        passes_++;
+        if (na == 0) return c_.P[pcur_];
+        pcur_ ^= 1;
+        double *P = c_.P[pcur_];
+        if (!timed(c_, ORTH_K_REDUCE, st_, (double)nwg * na * nb, 8.0 * ((double)nwg + 1) * na * nb,
+                   [&] { return launch_reduce(c_.part, nwg, na, nb, P, na, st_); }))
+            return nullptr;
+        syncs_++;
+        return P;
+    }
+
+    bool flush()
+    {
+        if (pend_.ngrp == 0) return true;
+        return pass(0, 0, 0, 0) != nullptr;
+    }
+
+    // Can every pass of this call run fused?  Widest tile: all n columns;
+    // largest coefficient block: (n + s) x n1.
+    bool fused_fits() const
+    {
+        PassArgs pa;
+        memset(&pa, 0, sizeof(pa));
+        pa.ncols = (int)n_;
+        pa.wl = (int)((n_ + 15) / 16 * 16);
+        pa.k1 = (int)std::min<int64_t>(n_, pa.wl);
+        pa.n1 = n1_;
+        const int64_t Mt = (n_ + 15) / 16;
+        pa.na = (int)std::min<int64_t>(n_, Mt * 16);
+        pa.nb = (int)std::min<int64_t>(n_, 2 * s_);
+        pa.a0 = 0;
+        pa.b0 = (int)(pa.wl - ((pa.nb + 15) / 16) * 16);
+        return pass_lds_bytes(pa) != 0;
+    }
+
+    bool intra_qr_f(int64_t blk, int64_t col0, int64_t w, double *Rdst, int64_t ldRdst)
+    {
+        const int ops[3] = {OP_CQR1, OP_CQR2, OP_CQR3};
+        for (int k = 0; k < 3; k++) {
+            double *P = pass(col0, w, col0, w);
+            if (!P) return false;
+            SmallArgs a = sa(ops[k], 0, blk, 0, w, 0, P, w);
+            a.Rdst = Rdst;
+            a.ldRdst = ldRdst;
+            if (!small(a)) return false;
+            pend_group(col0, w, 0, 0, Cq_ + k * kLdT * kLdT);
+        }
+        return true;
+    }
+
+    void pend_close_open(int64_t c, int64_t wb, int64_t wn, bool next_T, bool seq)
+    {
+        pend_.k1 = (int)(c + wb);
+        pend_.seq = seq ? 1 : 0;
+        pend_group(c, wb, 0, c, Om_);
+        if (wn > 0) pend_group(c + wb, wn, S_, c + wb, next_T ? Sig_ : nullptr);
+    }
+
+    bool p1s_f()
+    {
+        if (!intra_qr_f(0, 0, s_, Rd_, ldR_)) return false;
+        if (N_ == 1) return flush();
+        const int64_t w1 = bw(1);
+        double *P = pass(0, s_ + w1, s_, w1);
+        if (!P || !small(sa(OP_OPEN_PYTH, 0, 1, s_, w1, 0, P, s_ + w1))) return false;
+        pend_.k1 = (int)s_;
+        pend_group(s_, w1, 0, s_, Sig_);
+        int64_t c = s_;
+        for (int64_t b = 1; b < N_; b++) {
+            const int64_t wb = bw(b), wn = (b + 1 < N_) ? bw(b + 1) : 0;
+            const int64_t na = c + wb + wn;
+            P = pass(0, na, c, wb + wn);
+            if (!P || !small(sa(OP_CLOSE_OPEN, F_PYTH_NEXT, b, c, wb, wn, P, na))) return false;
+            pend_close_open(c, wb, wn, true, false);
+            c += wb;
+        }
+        return flush();
+    }
+
+    bool a1s_f()
+    {
+        if (!intra_qr_f(0, 0, s_, Rd_, ldR_)) return false;
+        if (N_ == 1) return flush();
+        const int64_t w1 = bw(1);
+        double *P = pass(0, s_, s_, w1);
+        if (!P || !small(sa(OP_PLAIN_S, 0, 1, s_, w1, 0, P, s_))) return false;
+        pend_.k1 = (int)s_;
+        pend_group(s_, w1, 0, s_, nullptr);
+        int64_t c = s_;
+        for (int64_t b = 1; b < N_; b++) {
+            const int64_t wb = bw(b), wn = (b + 1 < N_) ? bw(b + 1) : 0;
+            const int64_t na = c + wb;
+            P = pass(0, na, c, wb + wn);
+            if (!P || !small(sa(OP_CLOSE_OPEN, F_1S | F_SEQ, b, c, wb, wn, P, na))) return false;
+            pend_close_open(c, wb, wn, false, true);
+            c += wb;
+        }
+        return flush();
+    }
+
+    bool pipi_f()
+    {
+        if (!intra_qr_f(0, 0, s_, Rd_, ldR_)) return false;
+        int64_t c = s_;
+        for (int64_t b = 1; b < N_; b++) {
+            const int64_t wb = bw(b), na = c + wb;
+            double *P = pass(0, na, c, wb);
+            if (!P || !small(sa(OP_PIPI1, 0, b, c, wb, 0, P, na))) return false;
+            pend_.k1 = (int)c;
+            pend_group(c, wb, 0, c, Sig_);
+            P = pass(0, na, c, wb);
+            if (!P || !small(sa(OP_PIPI2, 0, b, c, wb, 0, P, na))) return false;
+            pend_.k1 = (int)c;
+            pend_group(c, wb, 0, c, Om_);
+            c += wb;
+        }
+        return flush();
+    }
+
+    bool i2_f()
+    {
+        if (!intra_qr_f(0, 0, s_, Rd_, ldR_)) return false;
+        int64_t c = s_;
+        for (int64_t b = 1; b < N_; b++) {
+            const int64_t wb = bw(b);
+            double *P = pass(0, c, c, wb);
+            if (!P || !small(sa(OP_PLAIN_S, 0, b, c, wb, 0, P, c))) return false;
+            pend_.k1 = (int)c;
+            pend_group(c, wb, 0, c, nullptr);
+            if (!intra_qr_f(b, c, wb, Sig_, kLdT)) return false;
+            P = pass(0, c, c, wb);
+            if (!P || !small(sa(OP_I2_W, 0, b, c, wb, 0, P, c))) return false;
+            pend_.k1 = (int)c;
+            pend_group(c, wb, 0, c, nullptr);
+            if (!intra_qr_f(b, c, wb, Om_, kLdT)) return false;
+            if (!small(sa(OP_RBB, 0, b, c, wb, 0, nullptr, 1))) return false;
+            c += wb;
+        }
+        return flush();
+    }
+
+    bool p2s_f()
+    {
+        if (!intra_qr_f(0, 0, s_, Rd_, ldR_)) return false;
+        if (N_ == 1) return flush();
+        const int64_t w1 = bw(1);
+        double *P = pass(0, s_, s_, w1);
+        if (!P || !small(sa(OP_PLAIN_S, 0, 1, s_, w1, 0, P, s_))) return false;
+        pend_.k1 = (int)s_;
+        pend_group(s_, w1, 0, s_, nullptr);
+        if (!intra_qr_f(1, s_, w1, Sig_, kLdT)) return false;
+        int64_t c = s_;
+        for (int64_t b = 1; b < N_; b++) {
+            const int64_t wb = bw(b), wn = (b + 1 < N_) ? bw(b + 1) : 0;
+            const int64_t na = c + wb;
+            P = pass(0, na, c, wb + wn);
+            if (!P || !small(sa(OP_CLOSE_OPEN, 0, b, c, wb, wn, P, na))) return false;
+            pend_close_open(c, wb, wn, false, false);
+            if (wn > 0 && !intra_qr_f(b + 1, c + wb, wn, Sig_, kLdT)) return false;
+            c += wb;
+        }
+        return flush();
+    }
+
     bool run(int variant)
     {
+        fused_ = !getenv("ORTH_UNFUSED") && fused_fits();
+        if (fused_) {
+            switch (variant) {
+            case ORTH_P1S: return p1s_f();
+            case ORTH_1S: return a1s_f();
+            case ORTH_PIPI: return pipi_f();
+            case ORTH_I2: return i2_f();
+            case ORTH_P2S: return p2s_f();
+            default: return false;
+            }
+        }
         switch (variant) {
         case ORTH_P1S: return p1s();
         case ORTH_1S: return a1s();
@@ -451,6 +731,8 @@ class Run {
 
     int passes_ = 0, syncs_ = 0;
     bool debug_ = getenv("ORTH_DEBUG_DUMP") != nullptr;
+    bool fused_ = false;
+    int n1_ = 32;
 
     // Debug aid (ORTH_DEBUG_DUMP set): print the reduced projection and the
     // factors after each small step.

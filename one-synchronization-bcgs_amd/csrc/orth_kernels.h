@@ -41,7 +41,8 @@ enum SmallOp {
 };
 enum SmallFlag {
     F_PYTH_NEXT = 1,  // next block's Sigma from the Pythagorean Gram (P1s)
-    F_1S = 2          // R(0:c,b) = S + Y and R_bb = Omega (BCGSI+A-1S)
+    F_1S = 2,         // R(0:c,b) = S + Y and R_bb = Omega (BCGSI+A-1S)
+    F_SEQ = 4         // emit the next pass for sequential mode (no folding)
 };
 
 struct SmallArgs {
@@ -57,9 +58,15 @@ struct SmallArgs {
     double *R;
     int64_t ldR;
     double *Sig, *Om, *Cq, *Ctmp;  // s x s stashes, ld kLdT (Cq holds 3)
+    double *Tbig;                  // (n + kMaxS) x kMaxS scratch
     double *Rdst;
     int64_t ldRdst;
     int *status;  // [0] = first breakdown block (1-based), [1] = pivot
+    // Coefficients of the next fused pass (row-major, ld n1), written by
+    // steps that open or close a block when Cout != nullptr.
+    double *Cout;
+    int n1;
+    int S;  // block template width: column offset of the second Z group
 };
 
 hipError_t launch_update(int S, double *X, int64_t ldX, int64_t m,
@@ -84,6 +91,49 @@ hipError_t launch_reduce(const double *part, int nwg, int64_t na, int64_t nb,
                          double *out, int64_t ldo, hipStream_t st);
 
 hipError_t launch_small(const SmallArgs &a, hipStream_t st);
+
+// ---------------------------------------------------------------------------
+// Fused streaming pass: one sweep over the row slab that applies the
+// row-local block updates and accumulates the next projection.
+//
+// Per 32-row tile staged in LDS (columns [0, ncols) of X):
+//   step 1  Z = tile(:, 0:k1) * C on FP64 MFMA (C is k1 x n1, row-major);
+//   groups  tile(:, t0:t0+w) -= Z(:, zoff:zoff+w), then a row-wise forward
+//           substitution with the upper factor T (if any); the groups are
+//           written back to X;
+//   step 2  partial += tile(:, a0:a0+na)^T tile(:, b0:b0+nb) on FP64 MFMA.
+// ---------------------------------------------------------------------------
+struct PassGroup {
+    int t0;          // first column of the target block
+    int w;           // its width (<= S)
+    int zoff;        // first column of Z subtracted from it
+    int k;           // inner dimension of its product in sequential mode
+    const double *T; // w x w upper factor (ld kLdT) or nullptr
+};
+
+struct PassArgs {
+    double *X;
+    int64_t ldX;
+    int64_t m;
+    int ncols;  // columns of X staged per tile
+    int wl;     // LDS tile width: multiple of 16 covering every range
+    int k1;     // inner dimension of step 1 (0: no product)
+    int n1;     // columns of Z (multiple of 16)
+    const double *C;
+    int ngrp;
+    int seq;  // 1: group 1's product uses group 0's finished columns
+    PassGroup g[2];
+    int a0, na, b0, nb;  // step 2 ranges (na == 0: no projection)
+    double *part;
+    int64_t rows_per;
+};
+
+constexpr int kPassRows = 32;
+constexpr int kPassMaxCols = 288;
+
+// LDS bytes the fused pass needs, or 0 when the shape does not fit it.
+size_t pass_lds_bytes(const PassArgs &a);
+hipError_t launch_pass(int S, const PassArgs &a, int nwg, hipStream_t st);
 
 // Zero n entries with a kernel (no memset nodes, so the sequence can be
 // captured in a graph).

@@ -11,6 +11,7 @@
 #include "orth_kernels.h"
 
 #include <math.h>
+#include <stdlib.h>
 
 namespace orth {
 
@@ -258,7 +259,7 @@ __device__ void d_tn(int k, int ma, const double *A, int64_t lda, int nb, const 
         const int i = e % ma, j = e / ma;
         double s = 0.0;
         for (int l = 0; l < k; l++) s = fma(A[l + i * lda], B[l + j * ldb], s);
-        C[i + j * ldc] = beta * C[i + j * ldc] + alpha * s;
+        C[i + j * ldc] = (beta == 0.0 ? 0.0 : beta * C[i + j * ldc]) + alpha * s;
     }
     __syncthreads();
 }
@@ -271,7 +272,7 @@ __device__ void d_nn(int ma, int k, const double *A, int64_t lda, int nb, const 
         const int i = e % ma, j = e / ma;
         double s = 0.0;
         for (int l = 0; l < k; l++) s = fma(A[i + l * lda], B[l + j * ldb], s);
-        C[i + j * ldc] = beta * C[i + j * ldc] + alpha * s;
+        C[i + j * ldc] = (beta == 0.0 ? 0.0 : beta * C[i + j * ldc]) + alpha * s;
     }
     __syncthreads();
 }
@@ -347,6 +348,44 @@ __device__ void d_trmm(int w, const double *A, const double *B, double *C, int64
     __syncthreads();
 }
 
+// Ui = U^{-1} for upper-triangular U (both ld kLdT), one column per thread.
+__device__ void d_trinv(int w, const double *U, double *Ui)
+{
+    for (int j = threadIdx.x; j < w; j += blockDim.x) {
+        for (int i = w - 1; i >= 0; i--) {
+            if (i > j) {
+                Ui[i + j * kLdT] = 0.0;
+                continue;
+            }
+            double t = (i == j) ? 1.0 : 0.0;
+            for (int l = i + 1; l <= j; l++) t = fma(-U[i + l * kLdT], Ui[l + j * kLdT], t);
+            Ui[i + j * kLdT] = t / U[i + i * kLdT];
+        }
+    }
+    __syncthreads();
+}
+
+// Coefficients of the next fused pass, row-major C[k][n] (ld n1) for
+// k < roundup4(k1): the first Z group is A(:, 0:w0) and comes from G0
+// (column-major, ld ld0, k0 rows, zero below); the optional second group
+// starts at column S and is given by G1 (k1 rows, ld ld1).
+__device__ void d_emit(double *C, int n1, int S, int k1, int k0, int w0, const double *G0,
+                       int64_t ld0, int w1, const double *G1, int64_t ld1)
+{
+    const int k1p = (k1 + 3) & ~3;
+    for (int e = threadIdx.x; e < k1p * n1; e += blockDim.x) {
+        const int k = e / n1, n = e - k * n1;
+        double v = 0.0;
+        if (n < w0) {
+            if (k < k0) v = G0[k + n * ld0];
+        } else if (n >= S && n < S + w1 && G1) {
+            if (k < k1) v = G1[k + (n - S) * ld1];
+        }
+        C[e] = v;
+    }
+    __syncthreads();
+}
+
 __device__ void note_info(int *status, int blk, int info)
 {
     if (threadIdx.x == 0 && info != 0 && status[0] == 0) {
@@ -367,6 +406,7 @@ __global__ __launch_bounds__(256) void k_small(SmallArgs a)
         d_tn(c, wb, P, na, wb, P, na, P + c, na, -1.0, 1.0);
         d_copy(c, wb, P, na, Rb, a.ldR);
         note_info(a.status, a.blk, d_chol(wb, P + c, na, 0.0, a.Sig));
+        if (a.Cout) d_emit(a.Cout, a.n1, a.S, c, c, wb, P, na, 0, nullptr, 0);
         break;
     }
     case OP_CLOSE_OPEN: {
@@ -399,6 +439,28 @@ __global__ __launch_bounds__(256) void k_small(SmallArgs a)
                 note_info(a.status, a.blk + 1, d_chol(wn, H, na, 0.0, a.Sig));
             }
         }
+        if (a.Cout && (a.flags & F_SEQ)) {
+            // Sequential next pass: group 1 uses S_n on the finished Q_b.
+            d_emit(a.Cout, a.n1, a.S, c + wb, c, wb, P, na, wn, P + (int64_t)wb * na, na);
+        } else if (a.Cout) {
+            // Next pass: group 0 finalizes block b with [Y; 0] and Omega; group 1
+            // is the first pass of block b+1 with the prefix [Q Q^_b] and
+            // [S_top - Y Omega^{-1} S_bot; Omega^{-1} S_bot], which equals
+            // Q S_top + Q_b S_bot without waiting for Q_b.
+            double *G1 = nullptr;
+            if (wn > 0) {
+                double *Oi = a.Ctmp, *T2 = a.Tbig;  // T2: (c+wb) x wn, ld c+wb
+                double *Pn = P + (int64_t)wb * na;
+                d_trinv(wb, a.Om, Oi);
+                // T2 rows c..c+wb-1 = Omega^{-1} S_bot.
+                d_nn(wb, wb, Oi, kLdT, wn, Pn + c, na, T2 + c, c + wb, 1.0, 0.0);
+                // T2 rows 0..c-1 = S_top - Y (Omega^{-1} S_bot).
+                d_copy(c, wn, Pn, na, T2, c + wb);
+                d_nn(c, wb, P, na, wn, T2 + c, c + wb, T2, c + wb, -1.0, 1.0);
+                G1 = T2;
+            }
+            d_emit(a.Cout, a.n1, a.S, c + wb, c, wb, P, na, wn, G1, c + wb);
+        }
         break;
     }
     case OP_PIPI1: {
@@ -406,6 +468,7 @@ __global__ __launch_bounds__(256) void k_small(SmallArgs a)
         d_tn(c, wb, P, na, wb, P, na, P + c, na, -1.0, 1.0);
         note_info(a.status, a.blk, d_chol(wb, P + c, na, 0.0, a.Sig));
         d_copy(c, wb, P, na, Rb, a.ldR);
+        if (a.Cout) d_emit(a.Cout, a.n1, a.S, c, c, wb, P, na, 0, nullptr, 0);
         break;
     }
     case OP_PIPI2: {
@@ -414,14 +477,17 @@ __global__ __launch_bounds__(256) void k_small(SmallArgs a)
         note_info(a.status, a.blk, d_chol(wb, P + c, na, 0.0, a.Om));
         d_nn(c, wb, P, na, wb, a.Sig, kLdT, Rb, a.ldR, 1.0, 1.0);
         d_trmm(wb, a.Om, a.Sig, Rbb, a.ldR);
+        if (a.Cout) d_emit(a.Cout, a.n1, a.S, c, c, wb, P, na, 0, nullptr, 0);
         break;
     }
     case OP_PLAIN_S: {
         d_copy(c, wb, P, na, Rb, a.ldR);
+        if (a.Cout) d_emit(a.Cout, a.n1, a.S, c, c, wb, P, na, 0, nullptr, 0);
         break;
     }
     case OP_I2_W: {
         d_nn(c, wb, P, na, wb, a.Sig, kLdT, Rb, a.ldR, 1.0, 1.0);
+        if (a.Cout) d_emit(a.Cout, a.n1, a.S, c, c, wb, P, na, 0, nullptr, 0);
         break;
     }
     case OP_RBB: {
@@ -461,6 +527,481 @@ hipError_t launch_small(const SmallArgs &a, hipStream_t st)
 {
     k_small<<<dim3(1), dim3(256), 0, st>>>(a);
     return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Fused pass: updates + projection in one sweep over the row slab
+// ---------------------------------------------------------------------------
+
+constexpr int kPT = kPassRows;                   // rows per tile (32)
+constexpr int kPLd = kPT + 1;                    // padded column stride (doubles)
+constexpr int kPF = kPassMaxCols * kPT / 256;    // prefetch registers per thread
+constexpr int kPassMaxTiles = 12;                // projection tiles per wave
+constexpr int kPassMaxC = 8192;                  // doubles of C held in LDS
+
+size_t pass_lds_bytes(const PassArgs &a)
+{
+    if (a.ncols > kPassMaxCols || a.wl < a.ncols || a.wl > kPassMaxCols + 32) return 0;
+    const int k1p = (a.k1 + 3) & ~3;
+    if ((size_t)k1p * a.n1 > (size_t)kPassMaxC || k1p > a.wl) return 0;
+    if (a.k1 > 0 && (a.n1 % 16 != 0 || (kPT / 16) * (a.n1 / 16) > 8)) return 0;
+    if (a.na > 0) {
+        const int Ma = (a.na + 15) / 16, Nb = (a.nb + 15) / 16;
+        if (Ma * Nb > 4 * kPassMaxTiles) return 0;
+        if (a.a0 + Ma * 16 > a.wl || a.b0 + Nb * 16 > a.wl) return 0;
+    }
+    for (int g = 0; g < a.ngrp; g++)
+        if (a.g[g].w > kMaxS || a.g[g].t0 + a.g[g].w > a.ncols) return 0;
+    return ((size_t)a.wl * kPLd + (size_t)k1p * a.n1 + 2 * kMaxS * kMaxS) * sizeof(double);
+}
+
+template <int S>
+__global__ __launch_bounds__(256) void k_pass(PassArgs a)
+{
+    extern __shared__ double sm[];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int li = lane & 15, lk = lane >> 4;
+    const int k1p = (a.k1 + 3) & ~3;
+    double *tile = sm;
+    double *Cs = tile + a.wl * kPLd;
+    double *Ts = Cs + k1p * a.n1;  // one S x S factor per group, ld S
+
+    for (int e = tid; e < k1p * a.n1; e += 256) Cs[e] = a.C[e];
+    for (int g = 0; g < a.ngrp; g++) {
+        const double *T = a.g[g].T;
+        const int w = a.g[g].w;
+        if (T)
+            for (int e = tid; e < S * S; e += 256) {
+                const int i = e % S, j = e / S;
+                Ts[g * S * S + e] = (i < w && j < w) ? T[i + j * kLdT] : 0.0;
+            }
+    }
+    for (int e = tid; e < (a.wl - a.ncols) * kPT; e += 256)
+        tile[(a.ncols + e / kPT) * kPLd + (e % kPT)] = 0.0;
+
+    const int Ma = (a.na + 15) >> 4, Nb = (a.nb + 15) >> 4;
+    const int ntiles = a.na > 0 ? Ma * Nb : 0;
+
+    dbl4 acc[kPassMaxTiles];
+#pragma unroll
+    for (int t = 0; t < kPassMaxTiles; t++) acc[t] = dbl4{0.0, 0.0, 0.0, 0.0};
+
+    const int64_t rb = (int64_t)blockIdx.x * a.rows_per;
+    const int64_t re = min(a.m, rb + a.rows_per);
+    const int nload = a.ncols * kPT;
+    double pf[kPF];
+    // Element e of a tile is column e / 32, row e % 32: each half-wave reads
+    // 32 consecutive rows (256 bytes) of one column.
+    auto load = [&](int64_t r0) {
+#pragma unroll
+        for (int i = 0; i < kPF; i++) {
+            const int e = tid + 256 * i;
+            if (e < nload) {
+                const int64_t row = r0 + (e & (kPT - 1));
+                pf[i] = row < re ? a.X[row + (int64_t)(e / kPT) * a.ldX] : 0.0;
+            }
+        }
+    };
+    if (rb < re) load(rb);
+    __syncthreads();
+
+    for (int64_t r0 = rb; r0 < re; r0 += kPT) {
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < kPF; i++) {
+            const int e = tid + 256 * i;
+            if (e < nload) tile[(e / kPT) * kPLd + (e & (kPT - 1))] = pf[i];
+        }
+        __syncthreads();
+        if (r0 + kPT < re) load(r0 + kPT);
+
+        if (a.ngrp > 0) {
+            // Step 1 in one phase (all groups from the same product) or, in
+            // sequential mode, one phase per group.
+            const int nphase = a.seq ? a.ngrp : 1;
+            for (int p = 0; p < nphase; p++) {
+                const int glo = a.seq ? p : 0, ghi = a.seq ? p + 1 : a.ngrp;
+                int zlo = 1 << 30, zhi = 0;
+                for (int g = glo; g < ghi; g++) {
+                    zlo = min(zlo, a.g[g].zoff);
+                    zhi = max(zhi, a.g[g].zoff + a.g[g].w);
+                }
+                const int ntlo = zlo >> 4, nN = ((zhi + 15) >> 4) - ntlo;
+                const int kp = a.seq ? ((a.g[p].k + 3) & ~3) : k1p;
+                const int ncombo = a.k1 > 0 ? (kPT / 16) * nN : 0;
+                // Z = tile(:, 0:kp) C(:, N-tiles), one 16 x 16 tile of Z per combo.
+                dbl4 z[2];
+#pragma unroll
+                for (int q = 0; q < 2; q++) {
+                    z[q] = dbl4{0.0, 0.0, 0.0, 0.0};
+                    const int combo = wave + 4 * q;
+                    if (combo < ncombo) {
+                        const int mt = combo / nN, nt = ntlo + combo % nN;
+                        const double *ap = tile + lk * kPLd + mt * 16 + li;
+                        const double *bp = Cs + lk * a.n1 + nt * 16 + li;
+                        for (int ks = 0; ks < kp / 4; ks++)
+                            z[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(
+                                ap[4 * ks * kPLd], bp[4 * ks * a.n1], z[q], 0, 0, 0);
+                    }
+                }
+                __syncthreads();
+                // Subtract Z from the target blocks (D layout: row lk + 4v, col li).
+#pragma unroll
+                for (int q = 0; q < 2; q++) {
+                    const int combo = wave + 4 * q;
+                    if (combo < ncombo) {
+                        const int mt = combo / nN, nt = ntlo + combo % nN;
+                        const int zc = nt * 16 + li;
+                        for (int g = glo; g < ghi; g++) {
+                            const int off = zc - a.g[g].zoff;
+                            if (off >= 0 && off < a.g[g].w) {
+                                double *col = tile + (a.g[g].t0 + off) * kPLd + mt * 16 + lk;
+#pragma unroll
+                                for (int v = 0; v < 4; v++) col[4 * v] -= z[q][v];
+                            }
+                        }
+                    }
+                }
+                __syncthreads();
+                // Row-wise forward substitution with each group's factor
+                // (dtrsm_('R','U','N','N') on the rows of the tile).
+                for (int task = tid; task < (ghi - glo) * kPT; task += 256) {
+                    const int g = glo + task / kPT, row = task % kPT;
+                    if (a.g[g].T) {
+                        const int w = a.g[g].w;
+                        double *col = tile + a.g[g].t0 * kPLd + row;
+                        const double *Tg = Ts + g * S * S;
+                        double x[S];
+#pragma unroll
+                        for (int i = 0; i < S; i++) x[i] = i < w ? col[i * kPLd] : 0.0;
+#pragma unroll
+                        for (int i = 0; i < S; i++) {
+                            if (i < w) {
+                                double v = x[i];
+#pragma unroll
+                                for (int l = 0; l < i; l++) v = fma(-x[l], Tg[l + i * S], v);
+                                x[i] = v * (1.0 / Tg[i + i * S]);
+                            }
+                        }
+#pragma unroll
+                        for (int i = 0; i < S; i++)
+                            if (i < w) col[i * kPLd] = x[i];
+                    }
+                }
+                __syncthreads();
+            }
+            // Write the finished target blocks back to X.
+            for (int g = 0; g < a.ngrp; g++) {
+                const int t0 = a.g[g].t0;
+                for (int e = tid; e < a.g[g].w * kPT; e += 256) {
+                    const int cc = e / kPT, rr = e - cc * kPT;
+                    const int64_t row = r0 + rr;
+                    if (row < re) a.X[row + (int64_t)(t0 + cc) * a.ldX] = tile[(t0 + cc) * kPLd + rr];
+                }
+            }
+        }
+
+        // Step 2: projection partials.
+        if (ntiles > 0) {
+#pragma unroll
+            for (int ks = 0; ks < kPT / 4; ks++) {
+                const int rr = 4 * ks + lk;
+#pragma unroll
+                for (int t = 0; t < kPassMaxTiles; t++) {
+                    const int tl = wave + 4 * t;
+                    if (tl < ntiles) {
+                        const int ta = tl / Nb, tb = tl - ta * Nb;
+                        const double av = tile[(a.a0 + ta * 16 + li) * kPLd + rr];
+                        const double bv = tile[(a.b0 + tb * 16 + li) * kPLd + rr];
+                        acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[t], 0, 0, 0);
+                    }
+                }
+            }
+        }
+    }
+
+    if (ntiles > 0) {
+        double *out = a.part + (int64_t)blockIdx.x * a.na * a.nb;
+#pragma unroll
+        for (int t = 0; t < kPassMaxTiles; t++) {
+            const int tl = wave + 4 * t;
+            if (tl < ntiles) {
+                const int ta = tl / Nb, tb = tl - ta * Nb;
+                const int j = tb * 16 + li;
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const int i = ta * 16 + lk + 4 * v;
+                    if (i < a.na && j < a.nb) out[i + (int64_t)j * a.na] = acc[t][v];
+                }
+            }
+        }
+    }
+}
+
+// Block width 16, the benchmark shape: compile-time tile counts, 32-bit
+// LDS offsets, four independent MFMA chains in step 1.  Wave w computes
+// Z tile (rows 16 (w & 1), cols 16 (w >> 1)) in step 1 and projection tiles
+// [w MT, (w+1) MT) x [0, NB) in step 2.
+template <int MT, int NB>
+__global__ __launch_bounds__(256) void k_pass16(PassArgs a)
+{
+    constexpr int S = 16, N1 = 32, LD = kPLd;
+    extern __shared__ double sm[];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int li = lane & 15, lk = lane >> 4;
+    const int k1p = (a.k1 + 3) & ~3;
+    const int C0 = a.wl * LD, T0 = C0 + k1p * N1;
+
+    for (int e = tid; e < k1p * N1; e += 256) sm[C0 + e] = a.C[e];
+    for (int g = 0; g < a.ngrp; g++) {
+        const double *T = a.g[g].T;
+        const int w = a.g[g].w;
+        if (T)
+            for (int e = tid; e < S * S; e += 256) {
+                const int i = e % S, j = e / S;
+                sm[T0 + g * S * S + e] = (i < w && j < w) ? T[i + j * kLdT] : 0.0;
+            }
+    }
+    for (int e = tid; e < (a.wl - a.ncols) * kPT; e += 256)
+        sm[(a.ncols + e / kPT) * LD + (e % kPT)] = 0.0;
+
+    const int Ma = (a.na + 15) >> 4;
+    const bool proj = a.na > 0;
+    const int mt1 = wave & 1, nt1 = wave >> 1;
+    const int aoff1 = lk * LD + mt1 * 16 + li;
+    const int boff1 = C0 + lk * N1 + nt1 * 16 + li;
+    int aoff2[MT], boff2[NB];
+#pragma unroll
+    for (int i = 0; i < MT; i++) aoff2[i] = (a.a0 + (wave * MT + i) * 16 + li) * LD + lk;
+#pragma unroll
+    for (int j = 0; j < NB; j++) boff2[j] = (a.b0 + j * 16 + li) * LD + lk;
+
+    dbl4 acc[MT][NB];
+#pragma unroll
+    for (int i = 0; i < MT; i++)
+#pragma unroll
+        for (int j = 0; j < NB; j++) acc[i][j] = dbl4{0.0, 0.0, 0.0, 0.0};
+
+    const int64_t rb = (int64_t)blockIdx.x * a.rows_per;
+    const int64_t re = min(a.m, rb + a.rows_per);
+    const int nload = a.ncols * kPT;
+    const double *Xl = a.X + (tid & (kPT - 1));
+    double pf[kPF];
+    auto load = [&](int64_t r0) {
+        const bool full = r0 + kPT <= re;
+#pragma unroll
+        for (int i = 0; i < kPF; i++) {
+            const int e = tid + 256 * i;
+            if (e < nload) {
+                const int64_t off = r0 + (int64_t)(e >> 5) * a.ldX;
+                pf[i] = (full || r0 + (e & 31) < re) ? Xl[off] : 0.0;
+            }
+        }
+    };
+    if (rb < re) load(rb);
+
+    for (int64_t r0 = rb; r0 < re; r0 += kPT) {
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < kPF; i++) {
+            const int e = tid + 256 * i;
+            if (e < nload) sm[(e >> 5) * LD + (e & 31)] = pf[i];
+        }
+        __syncthreads();
+        if (r0 + kPT < re) load(r0 + kPT);
+
+        if (a.ngrp > 0) {
+            const int nphase = a.seq ? a.ngrp : 1;
+            for (int p = 0; p < nphase; p++) {
+                const int glo = a.seq ? p : 0, ghi = a.seq ? p + 1 : a.ngrp;
+                const int kp = a.seq ? ((a.g[p].k + 3) & ~3) : k1p;
+                const bool mine = a.k1 > 0 && (!a.seq || nt1 == (a.g[p].zoff >> 4));
+                dbl4 z = dbl4{0.0, 0.0, 0.0, 0.0};
+                if (mine) {
+                    dbl4 z1 = z, z2 = z, z3 = z;
+                    const int nks = kp >> 2;
+                    int ks = 0;
+                    for (; ks + 4 <= nks; ks += 4) {
+                        const int ao = aoff1 + 4 * ks * LD, bo = boff1 + 4 * ks * N1;
+                        const double x0 = sm[ao], x1 = sm[ao + 4 * LD], x2 = sm[ao + 8 * LD],
+                                     x3 = sm[ao + 12 * LD];
+                        const double c0 = sm[bo], c1 = sm[bo + 4 * N1], c2 = sm[bo + 8 * N1],
+                                     c3 = sm[bo + 12 * N1];
+                        z = __builtin_amdgcn_mfma_f64_16x16x4f64(x0, c0, z, 0, 0, 0);
+                        z1 = __builtin_amdgcn_mfma_f64_16x16x4f64(x1, c1, z1, 0, 0, 0);
+                        z2 = __builtin_amdgcn_mfma_f64_16x16x4f64(x2, c2, z2, 0, 0, 0);
+                        z3 = __builtin_amdgcn_mfma_f64_16x16x4f64(x3, c3, z3, 0, 0, 0);
+                    }
+                    for (; ks < nks; ks++)
+                        z = __builtin_amdgcn_mfma_f64_16x16x4f64(sm[aoff1 + 4 * ks * LD],
+                                                                 sm[boff1 + 4 * ks * N1], z, 0,
+                                                                 0, 0);
+                    z = (z + z1) + (z2 + z3);
+                }
+                __syncthreads();
+                if (mine) {
+                    const int zc = nt1 * 16 + li;
+                    for (int g = glo; g < ghi; g++) {
+                        const int off = zc - a.g[g].zoff;
+                        if (off >= 0 && off < a.g[g].w) {
+                            const int o = (a.g[g].t0 + off) * LD + mt1 * 16 + lk;
+#pragma unroll
+                            for (int v = 0; v < 4; v++) sm[o + 4 * v] -= z[v];
+                        }
+                    }
+                }
+                __syncthreads();
+                // Row-wise forward substitution (dtrsm_('R','U','N','N')).
+                if (tid < (ghi - glo) * kPT) {
+                    const int g = glo + tid / kPT, row = tid % kPT;
+                    if (a.g[g].T) {
+                        const int w = a.g[g].w;
+                        const int o = a.g[g].t0 * LD + row;
+                        const int tg = T0 + g * S * S;
+                        double x[S];
+#pragma unroll
+                        for (int i = 0; i < S; i++) x[i] = i < w ? sm[o + i * LD] : 0.0;
+#pragma unroll
+                        for (int i = 0; i < S; i++) {
+                            if (i < w) {
+                                double v = x[i];
+#pragma unroll
+                                for (int l = 0; l < i; l++) v = fma(-x[l], sm[tg + l + i * S], v);
+                                x[i] = v * (1.0 / sm[tg + i + i * S]);
+                            }
+                        }
+#pragma unroll
+                        for (int i = 0; i < S; i++)
+                            if (i < w) sm[o + i * LD] = x[i];
+                    }
+                }
+                __syncthreads();
+            }
+            for (int g = 0; g < a.ngrp; g++) {
+                const int t0 = a.g[g].t0;
+                for (int e = tid; e < a.g[g].w * kPT; e += 256) {
+                    const int cc = e >> 5, rr = e & 31;
+                    if (r0 + rr < re)
+                        a.X[r0 + rr + (int64_t)(t0 + cc) * a.ldX] = sm[(t0 + cc) * LD + rr];
+                }
+            }
+        }
+
+        if (proj) {
+#pragma unroll
+            for (int ks = 0; ks < kPT / 4; ks++) {
+                double bv[NB], av[MT];
+#pragma unroll
+                for (int j = 0; j < NB; j++) bv[j] = sm[boff2[j] + 4 * ks];
+#pragma unroll
+                for (int i = 0; i < MT; i++) av[i] = sm[aoff2[i] + 4 * ks];
+#pragma unroll
+                for (int i = 0; i < MT; i++)
+#pragma unroll
+                    for (int j = 0; j < NB; j++)
+                        acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i], bv[j], acc[i][j],
+                                                                        0, 0, 0);
+            }
+        }
+    }
+
+    if (proj) {
+        double *out = a.part + (int64_t)blockIdx.x * a.na * a.nb;
+#pragma unroll
+        for (int i = 0; i < MT; i++) {
+            const int ta = wave * MT + i;
+            if (ta < Ma) {
+#pragma unroll
+                for (int j = 0; j < NB; j++) {
+                    const int col = j * 16 + li;
+#pragma unroll
+                    for (int v = 0; v < 4; v++) {
+                        const int row = ta * 16 + lk + 4 * v;
+                        if (row < a.na && col < a.nb) out[row + (int64_t)col * a.na] = acc[i][j][v];
+                    }
+                }
+            }
+        }
+    }
+}
+
+// Tiles per wave of the s = 16 kernel for a pass, or 0 when it does not apply.
+static int pass16_mt(const PassArgs &a)
+{
+    if (a.n1 != 32) return 0;
+    if (a.na == 0) return 1;
+    const int Ma = (a.na + 15) / 16, Nb = (a.nb + 15) / 16;
+    if (Nb > 2) return 0;
+    const int mt = (Ma + 3) / 4;
+    if (mt > 5 || a.a0 + 64 * mt > a.wl) return 0;
+    return mt;
+}
+
+template <int MT, int NB>
+hipError_t launch_pass16_t(const PassArgs &a, size_t lds, int nwg, hipStream_t st)
+{
+    static bool attr_set = false;
+    if (!attr_set) {
+        hipError_t e = hipFuncSetAttribute((const void *)k_pass16<MT, NB>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)(160 * 1024));
+        if (e != hipSuccess) return e;
+        attr_set = true;
+    }
+    k_pass16<MT, NB><<<dim3(nwg), dim3(256), lds, st>>>(a);
+    return hipGetLastError();
+}
+
+hipError_t launch_pass16(const PassArgs &a, size_t lds, int nwg, hipStream_t st)
+{
+    const int mt = pass16_mt(a);
+    const int nb = a.na > 0 ? (a.nb + 15) / 16 : 1;
+#define ORTH_P16(M)                                                                  \
+    case M:                                                                          \
+        return nb == 2 ? launch_pass16_t<M, 2>(a, lds, nwg, st)                     \
+                       : launch_pass16_t<M, 1>(a, lds, nwg, st);
+    switch (mt) {
+        ORTH_P16(1)
+        ORTH_P16(2)
+        ORTH_P16(3)
+        ORTH_P16(4)
+        ORTH_P16(5)
+    default: return hipErrorInvalidValue;
+    }
+#undef ORTH_P16
+}
+
+template <int S>
+hipError_t launch_pass_t(const PassArgs &a, int nwg, hipStream_t st)
+{
+    static bool attr_set = false;
+    const size_t lds = pass_lds_bytes(a);
+    if (lds == 0) return hipErrorInvalidValue;
+    if (!attr_set) {
+        hipError_t e = hipFuncSetAttribute((const void *)k_pass<S>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)(160 * 1024));
+        if (e != hipSuccess) return e;
+        attr_set = true;
+    }
+    k_pass<S><<<dim3(nwg), dim3(256), lds, st>>>(a);
+    return hipGetLastError();
+}
+
+hipError_t launch_pass(int S, const PassArgs &a, int nwg, hipStream_t st)
+{
+    if (S == 16 && pass16_mt(a) > 0 && !getenv("ORTH_GENERIC_PASS")) {
+        const size_t lds = pass_lds_bytes(a);
+        if (lds == 0) return hipErrorInvalidValue;
+        return launch_pass16(a, lds, nwg, st);
+    }
+    switch (S) {
+    case 4: return launch_pass_t<4>(a, nwg, st);
+    case 8: return launch_pass_t<8>(a, nwg, st);
+    case 16: return launch_pass_t<16>(a, nwg, st);
+    case 32: return launch_pass_t<32>(a, nwg, st);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 template <typename T>

@@ -202,3 +202,26 @@ def test_large_slab_properties(variant):
     assert st["info_block"] == 0
     assert loo < (1e-12 if variant == "bcgsi21s" else 1e-13)
     assert res / xn < 1e-15
+
+
+@pytest.mark.parametrize("variant", VARIANTS)
+@pytest.mark.parametrize("shape", [(4096, 64, 16), (3000, 37, 5), (2000, 96, 32)])
+def test_unfused_path_matches_oracle(variant, shape, monkeypatch):
+    """The separate update / projection kernels (used when a shape does not
+    fit the fused pass) stay at parity too."""
+    monkeypatch.setenv("ORTH_UNFUSED", "1")
+    m, n, s = shape
+    X = I.svd_matrix(m, n, 1e4, seed=29)
+    check_against_oracle(variant, X, s)
+
+
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_fused_and_unfused_agree(variant, monkeypatch):
+    X = I.svd_matrix(8192, 80, 1e5, seed=30)
+    say("case fused/unfused", variant)
+    Qf, Rf, stf = B.bcgs(variant, X, 16)
+    monkeypatch.setenv("ORTH_UNFUSED", "1")
+    Qu, Ru, stu = B.bcgs(variant, X, 16)
+    assert stf["passes"] < stu["passes"]
+    assert np.abs(Rf - Ru).max() <= 1e-11 * np.abs(Ru).max()
+    assert np.abs(Qf - Qu).max() <= 1e-10
