@@ -496,12 +496,6 @@ class Run {
             wl = std::max(wl, b0 - cb + ((nb + 15) / 16) * 16);
         }
         wl = std::max<int64_t>(wl, (pend_.k1 + 3) & ~3);
-        if (S_ == 16 && na > 0) {
-            // The s = 16 kernel gives each wave ceil(Ma / 4) projection tiles;
-            // the padding columns are zero.
-            const int64_t mt = ((na + 15) / 16 + 3) / 4;
-            wl = std::max(wl, a0 - cb + 64 * mt);
-        }
         wl = (wl + 15) / 16 * 16;
         pa.ncols = (int)ncols;
         pa.wl = (int)wl;
@@ -510,9 +504,21 @@ class Run {
         pa.b0 = (int)(b0 - cb);
         pa.nb = (int)nb;
         pa.part = c_.part;
+        // Kernel choice: the 512-thread s = 16 pass when it applies, else the
+        // 256-thread passes (whose s = 16 form needs the tile padded to
+        // ceil(Ma / 4) projection tiles per wave).
+        int mt = 0, nbt = 0;
+        const int ts16 = (S_ == 16 && !getenv("ORTH_PASS256")) ? pass16b_plan(pa, &mt, &nbt) : 0;
+        int64_t unit = kPassRows;
+        if (ts16 > 0) {
+            unit = ts16;
+        } else if (S_ == 16 && na > 0) {
+            const int64_t mtp = ((na + 15) / 16 + 3) / 4;
+            pa.wl = (int)std::max<int64_t>(pa.wl, a0 - cb + 64 * mtp);
+        }
         int nwg = std::max(1, c_.ncu);
         int64_t rows = (m_ + nwg - 1) / nwg;
-        rows = (rows + kPassRows - 1) / kPassRows * kPassRows;
+        rows = (rows + unit - 1) / unit * unit;
         pa.rows_per = rows;
         nwg = (int)((m_ + rows - 1) / rows);
         if (na > 0 && !ensure(c_.part, c_.part_n, (size_t)nwg * na * nb)) return nullptr;
@@ -522,8 +528,9 @@ class Run {
             flops += (double)m_ * pend_.g[g].w * (2.0 * kk + (pend_.g[g].T ? pend_.g[g].w : 0));
             wcols += pend_.g[g].w;
         }
-        if (!timed(c_, ORTH_K_PASS, st_, flops, 8.0 * m_ * (ncols + wcols),
-                   [&] { return launch_pass(S_, pa, nwg, st_); }))
+        if (!timed(c_, ORTH_K_PASS, st_, flops, 8.0 * m_ * (ncols + wcols), [&] {
+                return ts16 > 0 ? launch_pass16b(pa, nwg, st_) : launch_pass(S_, pa, nwg, st_);
+            }))
             return nullptr;
         pend_ = Pending();
         passes_++;

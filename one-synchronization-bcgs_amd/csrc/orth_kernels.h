@@ -135,6 +135,12 @@ constexpr int kPassMaxCols = 288;
 size_t pass_lds_bytes(const PassArgs &a);
 hipError_t launch_pass(int S, const PassArgs &a, int nwg, hipStream_t st);
 
+// Block width 16 with 512 threads per workgroup: super-tile rows for the
+// pass (multiple of 32, 0 when it does not apply); rows_per must be a
+// multiple of it.
+int pass16b_plan(const PassArgs &a, int *mt, int *nb);
+hipError_t launch_pass16b(const PassArgs &a, int nwg, hipStream_t st);
+
 // Zero n entries with a kernel (no memset nodes, so the sequence can be
 // captured in a graph).
 hipError_t launch_fill(double *p, size_t n, hipStream_t st);

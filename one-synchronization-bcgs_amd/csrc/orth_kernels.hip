@@ -925,6 +925,358 @@ __global__ __launch_bounds__(256) void k_pass16(PassArgs a)
     }
 }
 
+// ---------------------------------------------------------------------------
+// Fused pass, block width 16, 512 threads (two waves per SIMD).
+//
+// Rows are processed in super-tiles of TS = 32 f rows (f = 1, 2, 4, 8) so
+// that a super-tile holds ~10K doubles whatever the column count: the
+// prefetch of the next super-tile keeps ~80 KB of loads in flight per CU.
+// Step 1 (f == 1): wave w computes Z tile (w & 3) over half of the inner
+// dimension (w >> 2); the two halves are added in a fixed order through LDS.
+// Step 1 (f >= 2): wave w computes Z tiles (rows 16 ((w >> 1) + 4 j), cols
+// 16 (w & 1)).  Step 2: wave w owns projection N-tile (w & 1) and M-tiles
+// (w >> 1) + 4 j when nb > 16, else M-tiles w + 8 j.
+// ---------------------------------------------------------------------------
+
+constexpr int kB = 512;
+constexpr int kBTile = 10240;  // doubles of the LDS tile
+constexpr int kBPF = kBTile / kB;
+
+int pass16b_rows(const PassArgs &a)
+{
+    for (int f = 8; f >= 1; f >>= 1) {
+        const int ts = 32 * f;
+        if ((int64_t)a.wl * (ts + 1) <= kBTile && a.ncols * ts <= kBPF * kB) return ts;
+    }
+    return 0;
+}
+
+size_t pass16b_lds_bytes(const PassArgs &a, int ts)
+{
+    const int k1p = (a.k1 + 3) & ~3;
+    return ((size_t)a.wl * (ts + 1) + (size_t)k1p * 32 + 1024 + 2 * (16 * 16 + 16)) *
+           sizeof(double);
+}
+
+// Z tile of step 1 over k-steps [k0, k1e): A at sm[ao + 4 ks LD], B at
+// sm[bo + 4 ks N1].  Loads run one group of four k-steps ahead; the
+// prefetch index is clamped so that every path issues the same loads and
+// the waits stay partial.
+__device__ __forceinline__ dbl4 zsteps(const double *sm, int ao, int bo, int LD, int N1, int k0,
+                                       int k1e)
+{
+    dbl4 z = dbl4{0.0, 0.0, 0.0, 0.0};
+    int ks = k0;
+    if (k1e - k0 >= 4) {
+        double a0 = sm[ao + 4 * ks * LD], a1 = sm[ao + 4 * (ks + 1) * LD],
+               a2 = sm[ao + 4 * (ks + 2) * LD], a3 = sm[ao + 4 * (ks + 3) * LD];
+        double b0 = sm[bo + 4 * ks * N1], b1 = sm[bo + 4 * (ks + 1) * N1],
+               b2 = sm[bo + 4 * (ks + 2) * N1], b3 = sm[bo + 4 * (ks + 3) * N1];
+        for (; ks + 4 <= k1e; ks += 4) {
+            const double c0 = a0, c1 = a1, c2 = a2, c3 = a3;
+            const double d0 = b0, d1 = b1, d2 = b2, d3 = b3;
+            const int nx = min(ks + 4, k1e - 4);
+            a0 = sm[ao + 4 * nx * LD];
+            a1 = sm[ao + 4 * (nx + 1) * LD];
+            a2 = sm[ao + 4 * (nx + 2) * LD];
+            a3 = sm[ao + 4 * (nx + 3) * LD];
+            b0 = sm[bo + 4 * nx * N1];
+            b1 = sm[bo + 4 * (nx + 1) * N1];
+            b2 = sm[bo + 4 * (nx + 2) * N1];
+            b3 = sm[bo + 4 * (nx + 3) * N1];
+            z = __builtin_amdgcn_mfma_f64_16x16x4f64(c0, d0, z, 0, 0, 0);
+            z = __builtin_amdgcn_mfma_f64_16x16x4f64(c1, d1, z, 0, 0, 0);
+            z = __builtin_amdgcn_mfma_f64_16x16x4f64(c2, d2, z, 0, 0, 0);
+            z = __builtin_amdgcn_mfma_f64_16x16x4f64(c3, d3, z, 0, 0, 0);
+        }
+    }
+    for (; ks < k1e; ks++)
+        z = __builtin_amdgcn_mfma_f64_16x16x4f64(sm[ao + 4 * ks * LD], sm[bo + 4 * ks * N1], z, 0,
+                                                 0, 0);
+    return z;
+}
+
+template <int MT, int NB, int F>
+__global__ __launch_bounds__(kB) void k_pass16b(PassArgs a)
+{
+    constexpr int S = 16, N1 = 32, TS = 32 * F, LD = TS + 1;
+    constexpr int LTS = F == 1 ? 5 : F == 2 ? 6 : F == 4 ? 7 : 8;
+    extern __shared__ double sm[];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int li = lane & 15, lk = lane >> 4;
+    const int k1p = (a.k1 + 3) & ~3;
+    const int C0 = a.wl * LD, R0 = C0 + k1p * N1, T0 = R0 + 1024;
+
+    // Coefficients; factors column-major (ld S) with the reciprocal pivot on
+    // the diagonal (the dtrsm_ reference multiplies by 1 / T(j,j)).
+    for (int e = tid; e < k1p * N1; e += kB) sm[C0 + e] = a.C[e];
+    for (int g = 0; g < a.ngrp; g++) {
+        const double *T = a.g[g].T;
+        const int w = a.g[g].w;
+        if (T)
+            for (int e = tid; e < S * S; e += kB) {
+                const int i = e % S, j = e / S;
+                double v = 0.0;
+                if (i < w && j < w) v = (i == j) ? 1.0 / T[i + i * kLdT] : T[i + j * kLdT];
+                sm[T0 + g * S * S + e] = v;
+            }
+    }
+    for (int e = tid; e < (a.wl - a.ncols) * TS; e += kB)
+        sm[(a.ncols + (e >> LTS)) * LD + (e & (TS - 1))] = 0.0;
+
+    const int Ma = (a.na + 15) >> 4;
+    const bool proj = a.na > 0;
+    const int tb = NB == 2 ? (wave & 1) : 0;
+    const int mb = NB == 2 ? (wave >> 1) : wave;
+    constexpr int MS = NB == 2 ? 4 : 8;
+    // Tiles beyond Ma read zero columns or other valid tile columns; their
+    // products are never stored.
+    int aoff2[MT];
+#pragma unroll
+    for (int j = 0; j < MT; j++) {
+        const int ta = min(mb + MS * j, Ma - 1);
+        aoff2[j] = (a.a0 + ta * 16 + li) * LD + lk;
+    }
+    const int boff2 = (a.b0 + tb * 16 + li) * LD + lk;
+    const bool doproj = proj && tb * 16 < a.nb && mb < Ma;
+    dbl4 acc[MT];
+#pragma unroll
+    for (int j = 0; j < MT; j++) acc[j] = dbl4{0.0, 0.0, 0.0, 0.0};
+
+    const int64_t rb = (int64_t)blockIdx.x * a.rows_per;
+    const int64_t re = min(a.m, rb + a.rows_per);
+    // Thread tid moves row (tid % TS) of columns tid / TS + i (512 / TS).
+    constexpr int PSTEP = kB / TS;
+    const int prow = tid & (TS - 1), pcol = tid >> LTS;
+    const int nld = (a.ncols - pcol + PSTEP - 1) / PSTEP;
+    const double *px = a.X + prow + (int64_t)pcol * a.ldX;
+    const int64_t pxs = (int64_t)PSTEP * a.ldX;
+    const int pl = pcol * LD + prow;
+    double pf[kBPF];
+    auto load = [&](int64_t r0) {
+        const bool ok = r0 + prow < re;
+        const double *q = px + r0;
+#pragma unroll
+        for (int i = 0; i < kBPF; i++)
+            if (i < nld) pf[i] = ok ? q[i * pxs] : 0.0;
+    };
+    if (rb < re) load(rb);
+
+    for (int64_t r0 = rb; r0 < re; r0 += TS) {
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < kBPF; i++)
+            if (i < nld) sm[pl + i * PSTEP * LD] = pf[i];
+        __syncthreads();
+        if (r0 + TS < re) load(r0 + TS);
+
+        if (a.ngrp > 0) {
+            const int nphase = a.seq ? a.ngrp : 1;
+            for (int p = 0; p < nphase; p++) {
+                const int glo = a.seq ? p : 0, ghi = a.seq ? p + 1 : a.ngrp;
+                const int kp = a.seq ? ((a.g[p].k + 3) & ~3) : k1p;
+                const int nks = a.k1 > 0 ? kp >> 2 : 0;
+                if constexpr (F == 1) {
+                    // Z tile (wave & 3) over half (wave >> 2) of the k-steps;
+                    // the halves are added in a fixed order through LDS.
+                    const int item = wave & 3, mt = item & 1, nt = item >> 1, kh = wave >> 2;
+                    const bool mine = nks > 0 && (!a.seq || nt == (a.g[p].zoff >> 4));
+                    dbl4 z = dbl4{0.0, 0.0, 0.0, 0.0};
+                    if (mine) {
+                        const int half = nks >> 1;
+                        z = zsteps(sm, lk * LD + mt * 16 + li, C0 + lk * N1 + nt * 16 + li, LD, N1,
+                                   kh ? half : 0, kh ? nks : half);
+                        if (kh) {
+#pragma unroll
+                            for (int v = 0; v < 4; v++) sm[R0 + item * 256 + v * 64 + lane] = z[v];
+                        }
+                    }
+                    __syncthreads();
+                    if (mine && !kh) {
+#pragma unroll
+                        for (int v = 0; v < 4; v++) z[v] += sm[R0 + item * 256 + v * 64 + lane];
+                        const int zc = nt * 16 + li;
+                        for (int g = glo; g < ghi; g++) {
+                            const int off = zc - a.g[g].zoff;
+                            if (off >= 0 && off < a.g[g].w) {
+                                const int o = (a.g[g].t0 + off) * LD + mt * 16 + lk;
+#pragma unroll
+                                for (int v = 0; v < 4; v++) sm[o + 4 * v] -= z[v];
+                            }
+                        }
+                    }
+                } else {
+                    // Z tiles (rows 16 ((wave >> 1) + 4 j), cols 16 (wave & 1)).
+                    constexpr int NM = F > 1 ? F / 2 : 1;
+                    const int nt = wave & 1, mt0 = wave >> 1;
+                    const bool mine = nks > 0 && (!a.seq || nt == (a.g[p].zoff >> 4));
+                    dbl4 z[NM];
+#pragma unroll
+                    for (int j = 0; j < NM; j++) z[j] = dbl4{0.0, 0.0, 0.0, 0.0};
+                    if (mine) {
+                        const int bo = C0 + lk * N1 + nt * 16 + li;
+                        const int ao = lk * LD + mt0 * 16 + li;
+                        double bn = sm[bo], an[NM];
+#pragma unroll
+                        for (int j = 0; j < NM; j++) an[j] = sm[ao + 64 * j];
+                        for (int ks = 0; ks < nks; ks++) {
+                            const double bc = bn;
+                            double ac[NM];
+#pragma unroll
+                            for (int j = 0; j < NM; j++) ac[j] = an[j];
+                            const int nx = min(ks + 1, nks - 1);
+                            bn = sm[bo + 4 * nx * N1];
+#pragma unroll
+                            for (int j = 0; j < NM; j++) an[j] = sm[ao + 4 * nx * LD + 64 * j];
+#pragma unroll
+                            for (int j = 0; j < NM; j++)
+                                z[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ac[j], bc, z[j], 0, 0,
+                                                                           0);
+                        }
+                    }
+                    __syncthreads();
+                    if (mine) {
+                        const int zc = nt * 16 + li;
+                        for (int g = glo; g < ghi; g++) {
+                            const int off = zc - a.g[g].zoff;
+                            if (off >= 0 && off < a.g[g].w) {
+#pragma unroll
+                                for (int j = 0; j < NM; j++) {
+                                    const int o = (a.g[g].t0 + off) * LD + (mt0 + 4 * j) * 16 + lk;
+#pragma unroll
+                                    for (int v = 0; v < 4; v++) sm[o + 4 * v] -= z[j][v];
+                                }
+                            }
+                        }
+                    }
+                }
+                __syncthreads();
+                // Row-wise forward substitution in the operation order of
+                // dtrsm_('R','U','N','N'): x_j = (x_j - sum_{k<j} x_k T_kj) / T_jj,
+                // the row kept in the LDS tile.
+                for (int task = tid; task < (ghi - glo) * TS; task += kB) {
+                    const int g = glo + (task >> LTS), row = task & (TS - 1);
+                    if (a.g[g].T) {
+                        const int o = a.g[g].t0 * LD + row, tg = T0 + g * S * S;
+                        const int w = a.g[g].w;
+                        for (int j = 0; j < w; j++) {
+                            double v = sm[o + j * LD];
+                            for (int k = 0; k < j; k++) v = fma(-sm[o + k * LD], sm[tg + k + j * S], v);
+                            sm[o + j * LD] = v * sm[tg + j + j * S];
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+            if (r0 + prow < re) {
+                for (int g = 0; g < a.ngrp; g++) {
+                    const int t0 = a.g[g].t0;
+                    for (int cc = pcol; cc < a.g[g].w; cc += PSTEP)
+                        a.X[r0 + prow + (int64_t)(t0 + cc) * a.ldX] = sm[(t0 + cc) * LD + prow];
+                }
+            }
+        }
+
+        if (doproj) {
+#pragma unroll
+            for (int ks = 0; ks < TS / 4; ks++) {
+                const double bc = sm[boff2 + 4 * ks];
+                double ac[MT];
+#pragma unroll
+                for (int j = 0; j < MT; j++) ac[j] = sm[aoff2[j] + 4 * ks];
+#pragma unroll
+                for (int j = 0; j < MT; j++)
+                    acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ac[j], bc, acc[j], 0, 0, 0);
+            }
+        }
+    }
+
+    if (doproj) {
+        double *out = a.part + (int64_t)blockIdx.x * a.na * a.nb;
+#pragma unroll
+        for (int j = 0; j < MT; j++) {
+            const int ta = mb + MS * j;
+            if (ta < Ma) {
+                const int col = tb * 16 + li;
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const int row = ta * 16 + lk + 4 * v;
+                    if (row < a.na && col < a.nb) out[row + (int64_t)col * a.na] = acc[j][v];
+                }
+            }
+        }
+    }
+}
+
+template <int MT, int NB, int F>
+hipError_t launch_pass16b_t(const PassArgs &a, size_t lds, int nwg, hipStream_t st)
+{
+    static bool attr_set = false;
+    if (!attr_set) {
+        hipError_t e = hipFuncSetAttribute((const void *)k_pass16b<MT, NB, F>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)(160 * 1024));
+        if (e != hipSuccess) return e;
+        attr_set = true;
+    }
+    k_pass16b<MT, NB, F><<<dim3(nwg), dim3(kB), lds, st>>>(a);
+    return hipGetLastError();
+}
+
+template <int MT, int NB>
+hipError_t launch_pass16b_f(const PassArgs &a, int ts, size_t lds, int nwg, hipStream_t st)
+{
+    switch (ts) {
+    case 32: return launch_pass16b_t<MT, NB, 1>(a, lds, nwg, st);
+    case 64: return launch_pass16b_t<MT, NB, 2>(a, lds, nwg, st);
+    case 128: return launch_pass16b_t<MT, NB, 4>(a, lds, nwg, st);
+    case 256: return launch_pass16b_t<MT, NB, 8>(a, lds, nwg, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+// Super-tile rows of the 512-thread s = 16 pass, or 0 when it does not apply.
+int pass16b_plan(const PassArgs &a, int *mt, int *nb)
+{
+    if (a.n1 != 32 || (a.na > 0 && a.nb > 32)) return 0;
+    const int ts = pass16b_rows(a);
+    if (ts == 0) return 0;
+    if (pass16b_lds_bytes(a, ts) > 160 * 1024) return 0;
+    if (a.na > 0) {
+        const int Ma = (a.na + 15) / 16;
+        if (a.a0 + Ma * 16 > a.wl || a.b0 + ((a.nb + 15) / 16) * 16 > a.wl) return 0;
+        *nb = a.nb > 16 ? 2 : 1;
+        *mt = (Ma + (*nb == 2 ? 3 : 7)) / (*nb == 2 ? 4 : 8);
+        if (*mt > 4) return 0;
+    } else {
+        *nb = 1;
+        *mt = 1;
+    }
+    for (int g = 0; g < a.ngrp; g++)
+        if (a.g[g].w > 16 || a.g[g].t0 + a.g[g].w > a.ncols) return 0;
+    return ts;
+}
+
+hipError_t launch_pass16b(const PassArgs &a, int nwg, hipStream_t st)
+{
+    int mt = 0, nb = 0;
+    const int ts = pass16b_plan(a, &mt, &nb);
+    if (ts == 0) return hipErrorInvalidValue;
+    const size_t lds = pass16b_lds_bytes(a, ts);
+#define ORTH_P16B(M)                                                                 \
+    case M:                                                                          \
+        return nb == 2 ? launch_pass16b_f<M, 2>(a, ts, lds, nwg, st)                \
+                       : launch_pass16b_f<M, 1>(a, ts, lds, nwg, st);
+    switch (mt) {
+        ORTH_P16B(1)
+        ORTH_P16B(2)
+        ORTH_P16B(3)
+        ORTH_P16B(4)
+    default: return hipErrorInvalidValue;
+    }
+#undef ORTH_P16B
+}
+
 // Tiles per wave of the s = 16 kernel for a pass, or 0 when it does not apply.
 static int pass16_mt(const PassArgs &a)
 {
