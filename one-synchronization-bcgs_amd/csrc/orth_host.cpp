@@ -96,7 +96,7 @@ bool ctx_init(DevCtx &c)
 {
     if (c.init) return true;
     if (!ok(hipEventCreateWithFlags(&c.done, hipEventDisableTiming))) return false;
-    if (!ok(hipMalloc(&c.stash, 6 * kLdT * kLdT * sizeof(double)))) return false;
+    if (!ok(hipMalloc(&c.stash, 6 * kSlot * sizeof(double)))) return false;
     if (!ok(hipMalloc(&c.status_d, 4 * sizeof(int)))) return false;
     if (!ok(hipHostMalloc(&c.status_h, 4 * sizeof(int), hipHostMallocDefault))) return false;
     int dev = 0;
@@ -148,9 +148,9 @@ class Run {
         n1_ = std::max(16, 2 * S_);
         ldR_ = n;
         Sig_ = c.stash;
-        Om_ = Sig_ + kLdT * kLdT;
-        Cq_ = Om_ + kLdT * kLdT;
-        Ctmp_ = Cq_ + 3 * kLdT * kLdT;
+        Om_ = Sig_ + kSlot;
+        Cq_ = Om_ + kSlot;
+        Ctmp_ = Cq_ + 3 * kSlot;
     }
 
     bool prepare()
@@ -176,7 +176,7 @@ class Run {
         if (!ensure(c_.Tbig, c_.Tbig_n, (size_t)(n_ + kMaxS) * kMaxS)) return false;
         if (!ensure(c_.Cbuf, c_.Cbuf_n, (size_t)(n_ + 2 * kMaxS) * 2 * kMaxS)) return false;
         if (!ok(launch_fill(c_.Rd, rmax, st_))) return false;
-        if (!ok(launch_fill(c_.stash, 6 * kLdT * kLdT, st_))) return false;
+        if (!ok(launch_fill(c_.stash, 6 * kSlot, st_))) return false;
         if (!ok(launch_fill_int(c_.status_d, 4, st_))) return false;
         Rd_ = c_.Rd;
         return true;
@@ -295,7 +295,7 @@ class Run {
             a.Rdst = Rdst;
             a.ldRdst = ldRdst;
             if (!small(a)) return false;
-            if (!update(upd(col0, w, 0, nullptr, 0, Cq_ + k * kLdT * kLdT))) return false;
+            if (!update(upd(col0, w, 0, nullptr, 0, Cq_ + k * kSlot))) return false;
         }
         return true;
     }
@@ -578,7 +578,7 @@ class Run {
             a.Rdst = Rdst;
             a.ldRdst = ldRdst;
             if (!small(a)) return false;
-            pend_group(col0, w, 0, 0, Cq_ + k * kLdT * kLdT);
+            pend_group(col0, w, 0, 0, Cq_ + k * kSlot);
         }
         return true;
     }
@@ -746,7 +746,7 @@ class Run {
     void dump(const SmallArgs &a)
     {
         (void)hipStreamSynchronize(st_);
-        std::vector<double> h(6 * kLdT * kLdT);
+        std::vector<double> h(6 * kSlot);
         (void)hipMemcpy(h.data(), c_.stash, h.size() * sizeof(double), hipMemcpyDeviceToHost);
         int stat[4];
         (void)hipMemcpy(stat, c_.status_d, sizeof(stat), hipMemcpyDeviceToHost);
@@ -763,7 +763,7 @@ class Run {
         const char *names[6] = {"Sig", "Om", "Cq0", "Cq1", "Cq2", "Ctmp"};
         for (int k = 0; k < 6; k++) {
             fprintf(stderr, "  %s diag:", names[k]);
-            for (int i = 0; i < 4; i++) fprintf(stderr, " %.6e", h[k * kLdT * kLdT + i * (kLdT + 1)]);
+            for (int i = 0; i < 4; i++) fprintf(stderr, " %.6e", h[k * kSlot + i * (kLdT + 1)]);
             fprintf(stderr, "\n");
         }
     }

@@ -12,6 +12,9 @@ namespace orth {
 
 constexpr int kMaxS = 32;  // widest block the kernels support
 constexpr int kLdT = 32;   // leading dimension of every s x s factor
+// Each factor slot holds the 32 x 32 factor (ld kLdT) followed by the
+// reciprocals of its diagonal (written by the Cholesky step).
+constexpr int kSlot = kLdT * kLdT + kLdT;
 
 // One row-local block update of a streaming pass:
 //   X(:, col0:col0+w) <- (X(:, col0:col0+w) - X(:, 0:c) * W) * T^{-1}
@@ -57,7 +60,7 @@ struct SmallArgs {
     double *P;
     double *R;
     int64_t ldR;
-    double *Sig, *Om, *Cq, *Ctmp;  // s x s stashes, ld kLdT (Cq holds 3)
+    double *Sig, *Om, *Cq, *Ctmp;  // factor slots (kSlot apart; Cq holds 3)
     double *Tbig;                  // (n + kMaxS) x kMaxS scratch
     double *Rdst;
     int64_t ldRdst;

@@ -318,6 +318,10 @@ __device__ int d_chol(int w, const double *G, int64_t ldg, double shift, double 
         const int i = e % w, j = e / w;
         T[i + j * kLdT] = (i <= j) ? sh[i][j] : 0.0;
     }
+    // Reciprocal pivots after the factor (the dtrsm_ reference scales by
+    // 1 / T(j,j)); zero past w.
+    for (int i = threadIdx.x; i < kLdT; i += blockDim.x)
+        T[kLdT * kLdT + i] = i < w ? 1.0 / sh[i][i] : 0.0;
     const int info = info_s;
     __syncthreads();
     return info;
@@ -509,13 +513,13 @@ __global__ __launch_bounds__(256) void k_small(SmallArgs a)
         break;
     }
     case OP_CQR2: {
-        note_info(a.status, a.blk, d_chol(wb, P, na, 0.0, a.Cq + kLdT * kLdT));
+        note_info(a.status, a.blk, d_chol(wb, P, na, 0.0, a.Cq + kSlot));
         break;
     }
     case OP_CQR3: {
-        double *C2 = a.Cq + 2 * kLdT * kLdT;
+        double *C2 = a.Cq + 2 * kSlot;
         note_info(a.status, a.blk, d_chol(wb, P, na, 0.0, C2));
-        d_trmm(wb, C2, a.Cq + kLdT * kLdT, a.Ctmp, kLdT);
+        d_trmm(wb, C2, a.Cq + kSlot, a.Ctmp, kLdT);
         d_trmm(wb, a.Ctmp, a.Cq, a.Rdst, a.ldRdst);
         break;
     }
@@ -996,6 +1000,10 @@ __device__ __forceinline__ dbl4 zsteps(const double *sm, int ao, int bo, int LD,
     return z;
 }
 
+// Group fields by a runtime index without indexing the kernel-argument
+// array (which would force a private copy of the arguments).
+#define ORTH_G(args_, idx_, fld_) ((idx_) == 0 ? (args_).g[0].fld_ : (args_).g[1].fld_)
+
 template <int MT, int NB, int F>
 __global__ __launch_bounds__(kB) void k_pass16b(PassArgs a)
 {
@@ -1011,8 +1019,8 @@ __global__ __launch_bounds__(kB) void k_pass16b(PassArgs a)
     // the diagonal (the dtrsm_ reference multiplies by 1 / T(j,j)).
     for (int e = tid; e < k1p * N1; e += kB) sm[C0 + e] = a.C[e];
     for (int g = 0; g < a.ngrp; g++) {
-        const double *T = a.g[g].T;
-        const int w = a.g[g].w;
+        const double *T = ORTH_G(a, g, T);
+        const int w = ORTH_G(a, g, w);
         if (T)
             for (int e = tid; e < S * S; e += kB) {
                 const int i = e % S, j = e / S;
@@ -1074,13 +1082,13 @@ __global__ __launch_bounds__(kB) void k_pass16b(PassArgs a)
             const int nphase = a.seq ? a.ngrp : 1;
             for (int p = 0; p < nphase; p++) {
                 const int glo = a.seq ? p : 0, ghi = a.seq ? p + 1 : a.ngrp;
-                const int kp = a.seq ? ((a.g[p].k + 3) & ~3) : k1p;
+                const int kp = a.seq ? ((ORTH_G(a, p, k) + 3) & ~3) : k1p;
                 const int nks = a.k1 > 0 ? kp >> 2 : 0;
                 if constexpr (F == 1) {
                     // Z tile (wave & 3) over half (wave >> 2) of the k-steps;
                     // the halves are added in a fixed order through LDS.
                     const int item = wave & 3, mt = item & 1, nt = item >> 1, kh = wave >> 2;
-                    const bool mine = nks > 0 && (!a.seq || nt == (a.g[p].zoff >> 4));
+                    const bool mine = nks > 0 && (!a.seq || nt == (ORTH_G(a, p, zoff) >> 4));
                     dbl4 z = dbl4{0.0, 0.0, 0.0, 0.0};
                     if (mine) {
                         const int half = nks >> 1;
@@ -1097,9 +1105,9 @@ __global__ __launch_bounds__(kB) void k_pass16b(PassArgs a)
                         for (int v = 0; v < 4; v++) z[v] += sm[R0 + item * 256 + v * 64 + lane];
                         const int zc = nt * 16 + li;
                         for (int g = glo; g < ghi; g++) {
-                            const int off = zc - a.g[g].zoff;
-                            if (off >= 0 && off < a.g[g].w) {
-                                const int o = (a.g[g].t0 + off) * LD + mt * 16 + lk;
+                            const int off = zc - ORTH_G(a, g, zoff);
+                            if (off >= 0 && off < ORTH_G(a, g, w)) {
+                                const int o = (ORTH_G(a, g, t0) + off) * LD + mt * 16 + lk;
 #pragma unroll
                                 for (int v = 0; v < 4; v++) sm[o + 4 * v] -= z[v];
                             }
@@ -1109,7 +1117,7 @@ __global__ __launch_bounds__(kB) void k_pass16b(PassArgs a)
                     // Z tiles (rows 16 ((wave >> 1) + 4 j), cols 16 (wave & 1)).
                     constexpr int NM = F > 1 ? F / 2 : 1;
                     const int nt = wave & 1, mt0 = wave >> 1;
-                    const bool mine = nks > 0 && (!a.seq || nt == (a.g[p].zoff >> 4));
+                    const bool mine = nks > 0 && (!a.seq || nt == (ORTH_G(a, p, zoff) >> 4));
                     dbl4 z[NM];
 #pragma unroll
                     for (int j = 0; j < NM; j++) z[j] = dbl4{0.0, 0.0, 0.0, 0.0};
@@ -1138,11 +1146,11 @@ __global__ __launch_bounds__(kB) void k_pass16b(PassArgs a)
                     if (mine) {
                         const int zc = nt * 16 + li;
                         for (int g = glo; g < ghi; g++) {
-                            const int off = zc - a.g[g].zoff;
-                            if (off >= 0 && off < a.g[g].w) {
+                            const int off = zc - ORTH_G(a, g, zoff);
+                            if (off >= 0 && off < ORTH_G(a, g, w)) {
 #pragma unroll
                                 for (int j = 0; j < NM; j++) {
-                                    const int o = (a.g[g].t0 + off) * LD + (mt0 + 4 * j) * 16 + lk;
+                                    const int o = (ORTH_G(a, g, t0) + off) * LD + (mt0 + 4 * j) * 16 + lk;
 #pragma unroll
                                     for (int v = 0; v < 4; v++) sm[o + 4 * v] -= z[j][v];
                                 }
@@ -1152,17 +1160,37 @@ __global__ __launch_bounds__(kB) void k_pass16b(PassArgs a)
                 }
                 __syncthreads();
                 // Row-wise forward substitution in the operation order of
-                // dtrsm_('R','U','N','N'): x_j = (x_j - sum_{k<j} x_k T_kj) / T_jj,
-                // the row kept in the LDS tile.
-                for (int task = tid; task < (ghi - glo) * TS; task += kB) {
-                    const int g = glo + (task >> LTS), row = task & (TS - 1);
-                    if (a.g[g].T) {
-                        const int o = a.g[g].t0 * LD + row, tg = T0 + g * S * S;
-                        const int w = a.g[g].w;
-                        for (int j = 0; j < w; j++) {
-                            double v = sm[o + j * LD];
-                            for (int k = 0; k < j; k++) v = fma(-sm[o + k * LD], sm[tg + k + j * S], v);
-                            sm[o + j * LD] = v * sm[tg + j + j * S];
+                // dtrsm_('R','U','N','N'): x_j = (x_j - sum_{k<j} x_k T_kj) / T_jj.
+                // One group per wave, so the factor and its reciprocal pivots
+                // are wave-uniform and come through the scalar cache; the row
+                // lives in registers.
+                {
+                    constexpr int NBLK = TS > 64 ? TS / 64 : 1;
+                    for (int task = wave; task < (ghi - glo) * NBLK; task += kB / 64) {
+                        const int g = __builtin_amdgcn_readfirstlane(glo + task / NBLK);
+                        const double *Tg = ORTH_G(a, g, T);
+                        // The factor is read-only during the pass: read it
+                        // through the constant address space (scalar loads).
+                        typedef const __attribute__((address_space(4))) double cdouble;
+                        cdouble *T = (cdouble *)Tg;
+                        const int row = (task % NBLK) * 64 + lane;
+                        if (Tg && row < TS) {
+                            cdouble *rv = T + kLdT * kLdT;
+                            const int o = ORTH_G(a, g, t0) * LD + row;
+                            double x[S];
+#pragma unroll
+                            for (int i = 0; i < S; i++) x[i] = sm[o + i * LD];
+#pragma unroll
+                            for (int j = 0; j < S; j++) {
+                                double v = x[j];
+#pragma unroll
+                                for (int k = 0; k < j; k++) v = fma(-x[k], T[k + j * kLdT], v);
+                                x[j] = v * rv[j];
+                            }
+                            const int w = ORTH_G(a, g, w);
+#pragma unroll
+                            for (int i = 0; i < S; i++)
+                                if (i < w) sm[o + i * LD] = x[i];
                         }
                     }
                 }
@@ -1170,8 +1198,8 @@ __global__ __launch_bounds__(kB) void k_pass16b(PassArgs a)
             }
             if (r0 + prow < re) {
                 for (int g = 0; g < a.ngrp; g++) {
-                    const int t0 = a.g[g].t0;
-                    for (int cc = pcol; cc < a.g[g].w; cc += PSTEP)
+                    const int t0 = ORTH_G(a, g, t0);
+                    for (int cc = pcol; cc < ORTH_G(a, g, w); cc += PSTEP)
                         a.X[r0 + prow + (int64_t)(t0 + cc) * a.ldX] = sm[(t0 + cc) * LD + prow];
                 }
             }
