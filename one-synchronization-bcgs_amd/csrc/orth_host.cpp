@@ -516,7 +516,9 @@ class Run {
             const int64_t mtp = ((na + 15) / 16 + 3) / 4;
             pa.wl = (int)std::max<int64_t>(pa.wl, a0 - cb + 64 * mtp);
         }
-        int nwg = std::max(1, c_.ncu);
+        // Two 512-thread workgroups per CU when the s = 16 pass applies (its
+        // LDS is under half a CU's), so one hides the other's barriers.
+        int nwg = std::max(1, c_.ncu) * (ts16 > 0 ? 2 : 1);
         int64_t rows = (m_ + nwg - 1) / nwg;
         rows = (rows + unit - 1) / unit * unit;
         pa.rows_per = rows;

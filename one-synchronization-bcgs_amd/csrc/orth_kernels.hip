@@ -957,9 +957,7 @@ int pass16b_rows(const PassArgs &a)
 
 size_t pass16b_lds_bytes(const PassArgs &a, int ts)
 {
-    const int k1p = (a.k1 + 3) & ~3;
-    return ((size_t)a.wl * (ts + 1) + (size_t)k1p * 32 + 1024 + 2 * (16 * 16 + 16)) *
-           sizeof(double);
+    return ((size_t)a.wl * (ts + 1) + 1024) * sizeof(double);
 }
 
 // Z tile of step 1 over k-steps [k0, k1e): A at sm[ao + 4 ks LD], B at
@@ -1004,8 +1002,43 @@ __device__ __forceinline__ dbl4 zsteps(const double *sm, int ao, int bo, int LD,
 // array (which would force a private copy of the arguments).
 #define ORTH_G(args_, idx_, fld_) ((idx_) == 0 ? (args_).g[0].fld_ : (args_).g[1].fld_)
 
+// zsteps with the B operand (the coefficients) read from global memory.
+__device__ __forceinline__ dbl4 zsteps_g(const double *sm, const double *__restrict__ C, int ao,
+                                         int bo, int LD, int N1, int k0, int k1e)
+{
+    dbl4 z = dbl4{0.0, 0.0, 0.0, 0.0};
+    int ks = k0;
+    if (k1e - k0 >= 4) {
+        double b0 = C[bo + 4 * ks * N1], b1 = C[bo + 4 * (ks + 1) * N1],
+               b2 = C[bo + 4 * (ks + 2) * N1], b3 = C[bo + 4 * (ks + 3) * N1];
+        double a0 = sm[ao + 4 * ks * LD], a1 = sm[ao + 4 * (ks + 1) * LD],
+               a2 = sm[ao + 4 * (ks + 2) * LD], a3 = sm[ao + 4 * (ks + 3) * LD];
+        for (; ks + 4 <= k1e; ks += 4) {
+            const double c0 = a0, c1 = a1, c2 = a2, c3 = a3;
+            const double d0 = b0, d1 = b1, d2 = b2, d3 = b3;
+            const int nx = min(ks + 4, k1e - 4);
+            b0 = C[bo + 4 * nx * N1];
+            b1 = C[bo + 4 * (nx + 1) * N1];
+            b2 = C[bo + 4 * (nx + 2) * N1];
+            b3 = C[bo + 4 * (nx + 3) * N1];
+            a0 = sm[ao + 4 * nx * LD];
+            a1 = sm[ao + 4 * (nx + 1) * LD];
+            a2 = sm[ao + 4 * (nx + 2) * LD];
+            a3 = sm[ao + 4 * (nx + 3) * LD];
+            z = __builtin_amdgcn_mfma_f64_16x16x4f64(c0, d0, z, 0, 0, 0);
+            z = __builtin_amdgcn_mfma_f64_16x16x4f64(c1, d1, z, 0, 0, 0);
+            z = __builtin_amdgcn_mfma_f64_16x16x4f64(c2, d2, z, 0, 0, 0);
+            z = __builtin_amdgcn_mfma_f64_16x16x4f64(c3, d3, z, 0, 0, 0);
+        }
+    }
+    for (; ks < k1e; ks++)
+        z = __builtin_amdgcn_mfma_f64_16x16x4f64(sm[ao + 4 * ks * LD], C[bo + 4 * ks * N1], z, 0,
+                                                 0, 0);
+    return z;
+}
+
 template <int MT, int NB, int F>
-__global__ __launch_bounds__(kB) void k_pass16b(PassArgs a)
+__global__ __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_pass16b(PassArgs a)
 {
     constexpr int S = 16, N1 = 32, TS = 32 * F, LD = TS + 1;
     constexpr int LTS = F == 1 ? 5 : F == 2 ? 6 : F == 4 ? 7 : 8;
@@ -1013,22 +1046,11 @@ __global__ __launch_bounds__(kB) void k_pass16b(PassArgs a)
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int li = lane & 15, lk = lane >> 4;
     const int k1p = (a.k1 + 3) & ~3;
-    const int C0 = a.wl * LD, R0 = C0 + k1p * N1, T0 = R0 + 1024;
-
-    // Coefficients; factors column-major (ld S) with the reciprocal pivot on
-    // the diagonal (the dtrsm_ reference multiplies by 1 / T(j,j)).
-    for (int e = tid; e < k1p * N1; e += kB) sm[C0 + e] = a.C[e];
-    for (int g = 0; g < a.ngrp; g++) {
-        const double *T = ORTH_G(a, g, T);
-        const int w = ORTH_G(a, g, w);
-        if (T)
-            for (int e = tid; e < S * S; e += kB) {
-                const int i = e % S, j = e / S;
-                double v = 0.0;
-                if (i < w && j < w) v = (i == j) ? 1.0 / T[i + i * kLdT] : T[i + j * kLdT];
-                sm[T0 + g * S * S + e] = v;
-            }
-    }
+    // LDS: the row tile, then the scratch of the step-1 half-sum.  The
+    // coefficients C stay in global memory (one small block shared by every
+    // workgroup, so it is served from L2).
+    const int R0 = a.wl * LD;
+    const double *__restrict__ Cg = a.C;
     for (int e = tid; e < (a.wl - a.ncols) * TS; e += kB)
         sm[(a.ncols + (e >> LTS)) * LD + (e & (TS - 1))] = 0.0;
 
@@ -1092,8 +1114,8 @@ __global__ __launch_bounds__(kB) void k_pass16b(PassArgs a)
                     dbl4 z = dbl4{0.0, 0.0, 0.0, 0.0};
                     if (mine) {
                         const int half = nks >> 1;
-                        z = zsteps(sm, lk * LD + mt * 16 + li, C0 + lk * N1 + nt * 16 + li, LD, N1,
-                                   kh ? half : 0, kh ? nks : half);
+                        z = zsteps_g(sm, Cg, lk * LD + mt * 16 + li, lk * N1 + nt * 16 + li, LD, N1,
+                                     kh ? half : 0, kh ? nks : half);
                         if (kh) {
 #pragma unroll
                             for (int v = 0; v < 4; v++) sm[R0 + item * 256 + v * 64 + lane] = z[v];
@@ -1122,9 +1144,9 @@ __global__ __launch_bounds__(kB) void k_pass16b(PassArgs a)
 #pragma unroll
                     for (int j = 0; j < NM; j++) z[j] = dbl4{0.0, 0.0, 0.0, 0.0};
                     if (mine) {
-                        const int bo = C0 + lk * N1 + nt * 16 + li;
+                        const int bo = lk * N1 + nt * 16 + li;
                         const int ao = lk * LD + mt0 * 16 + li;
-                        double bn = sm[bo], an[NM];
+                        double bn = Cg[bo], an[NM];
 #pragma unroll
                         for (int j = 0; j < NM; j++) an[j] = sm[ao + 64 * j];
                         for (int ks = 0; ks < nks; ks++) {
@@ -1133,7 +1155,7 @@ __global__ __launch_bounds__(kB) void k_pass16b(PassArgs a)
 #pragma unroll
                             for (int j = 0; j < NM; j++) ac[j] = an[j];
                             const int nx = min(ks + 1, nks - 1);
-                            bn = sm[bo + 4 * nx * N1];
+                            bn = Cg[bo + 4 * nx * N1];
 #pragma unroll
                             for (int j = 0; j < NM; j++) an[j] = sm[ao + 4 * nx * LD + 64 * j];
 #pragma unroll
