@@ -61,16 +61,16 @@ def make_input(torch, m, n, kappa, seed):
     return X.contiguous()
 
 
-def cpu_baseline(n, s, kappa, rows, threads, m_full):
-    import numpy as np
-
+def cpu_baseline(X0, variant, n, s, rows, threads, m_full):
+    """Time the CPU oracle on the first `rows` rows of the device input X0
+    (the same bits the GPU factorizes), scaled linearly in m."""
     import inputs
     import oracle
 
     oracle.set_threads(threads)
-    X = inputs.svd_matrix(rows, n, kappa, seed=1)
+    X = X0[:, :rows].cpu().numpy().T  # rows x n, column-major
     t0 = time.perf_counter()
-    Q, R, info = oracle.bcgs("bcgsi2P1s", X, s)
+    Q, R, info = oracle.bcgs(variant, X, s)
     t = time.perf_counter() - t0
     loo = inputs.loo(Q)
     t_full = t * (m_full / rows)
@@ -79,11 +79,25 @@ def cpu_baseline(n, s, kappa, rows, threads, m_full):
         "unit": "GB/s",
         "cores": threads,
         "kind": "port",
-        "sample": ("CPU oracle bcgsi2P1s FP64 on m=%d rows (n=%d, s=%d, kappa=%g), "
-                   "%.2f s, scaled linearly in m to m=%d: %.1f ms per factorization; "
-                   "LOO %.1e" % (rows, n, s, kappa, t, m_full, t_full * 1e3, loo)),
+        "sample": ("CPU oracle (C restatement, OpenMP) %s FP64 on rows 0..%d of the same "
+                   "device input (n=%d, s=%d), %.2f s, scaled linearly in m to m=%d: "
+                   "%.1f ms per factorization; sample LOO %.1e" % (
+                       variant, rows, n, s, t, m_full, t_full * 1e3, loo)),
         "ms_per_factorization": round(t_full * 1e3, 1),
     }
+
+
+def pass_traffic(variant, m, n, s):
+    """HBM bytes per streaming-pass launch from the committed rocprofv3
+    counter summary of this workload (FETCH_SIZE + WRITE_SIZE, scaled by the
+    same counters read on a plain copy kernel), or None."""
+    path = os.path.join(ROOT, "profiles", "pass_traffic_%s_m%d_n%d_s%d.md" % (variant, m, n, s))
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
 
 
 def main():
@@ -96,10 +110,13 @@ def main():
     ap.add_argument("--s", type=int, default=16)
     ap.add_argument("--kappa", type=float, default=1e6)
     ap.add_argument("--variant", default="bcgsi2P1s")
-    ap.add_argument("--cpu-rows", type=int, default=1 << 19)
+    ap.add_argument("--cpu-rows", type=int, default=1 << 23)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="after timing, run one plain elementwise copy of X (read 8mn, "
+                         "write 8mn bytes) to calibrate HBM byte counters")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -147,6 +164,10 @@ def main():
     step_ms = [a.elapsed_time(b) for a, b in evs]
     ms = sum(step_ms) / len(step_ms)
     st = B.status()
+    if args.copy_probe:
+        torch.mul(X0, 1.0, out=X)
+        torch.cuda.synchronize()
+        log("copy probe: %d bytes read, %d bytes written" % (8 * m * n, 8 * m * n))
     log("steps ms:", ["%.2f" % v for v in step_ms], "wall %.3f s" % wall)
     log("kernel classes:", json.dumps(kt))
 
@@ -169,7 +190,8 @@ def main():
     else:
         roof = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4)}
-    roof.update({"traffic": None, "kernel": name, "launches": d["launches"],
+    traffic = pass_traffic(args.variant, m, n, s) if name == "pass" else None
+    roof.update({"traffic": traffic, "kernel": name, "launches": d["launches"],
                  "avg_launch_ms": round(d["ms"] / d["launches"], 4),
                  "share_of_step": round(d["ms"] / (ms * args.steps), 3),
                  "alg_gbs": round(gbs, 1), "alg_tflops": round(tfs, 3)})
@@ -202,7 +224,7 @@ def main():
     out.update(acc)
     if not args.no_cpu:
         log("CPU baseline ...")
-        out["cpu_baseline"] = cpu_baseline(n, s, args.kappa, args.cpu_rows,
+        out["cpu_baseline"] = cpu_baseline(X0, args.variant, n, s, min(args.cpu_rows, m),
                                            args.cpu_threads, m)
     print(json.dumps(out), flush=True)
 

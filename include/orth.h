@@ -7,8 +7,9 @@
  *
  *   m       global row count of X (one process: the local row count msub = m,
  *           the row split of SRC/orth.c:69-76 with one rank)
- *   n, s    number of columns and block width; the first block has s columns,
- *           the others s, the last one n - s*ceil((n-s)/s)... i.e. ragged
+ *   n, s    number of columns and block width: blocks of s columns from the
+ *           left, the last block holding the n mod s remaining columns when
+ *           s does not divide n
  *   Xsub    m x n column-major, leading dimension ldXsub; overwritten by Q
  *   R       n x n column-major, leading dimension ldR; for block column k the
  *           rows 0 .. (end of block k) are written, rows below are never
