@@ -12,9 +12,13 @@ namespace orth {
 
 constexpr int kMaxS = 32;  // widest block the kernels support
 constexpr int kLdT = 32;   // leading dimension of every s x s factor
-// Each factor slot holds the 32 x 32 factor (ld kLdT) followed by the
-// reciprocals of its diagonal (written by the Cholesky step).
-constexpr int kSlot = kLdT * kLdT + kLdT;
+// Each factor slot holds the 32 x 32 factor (ld kLdT), the reciprocals of
+// its diagonal, and the inverses of its 4 x 4 diagonal blocks (block b at
+// offset kSlotDiag + 16 b, column-major, zero past the factor's width), all
+// written by the Cholesky step.
+constexpr int kSlotRecip = kLdT * kLdT;
+constexpr int kSlotDiag = kSlotRecip + kLdT;
+constexpr int kSlot = kSlotDiag + (kMaxS / 4) * 16;
 
 // One row-local block update of a streaming pass:
 //   X(:, col0:col0+w) <- (X(:, col0:col0+w) - X(:, 0:c) * W) * T^{-1}

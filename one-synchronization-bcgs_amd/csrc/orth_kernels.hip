@@ -321,7 +321,23 @@ __device__ int d_chol(int w, const double *G, int64_t ldg, double shift, double 
     // Reciprocal pivots after the factor (the dtrsm_ reference scales by
     // 1 / T(j,j)); zero past w.
     for (int i = threadIdx.x; i < kLdT; i += blockDim.x)
-        T[kLdT * kLdT + i] = i < w ? 1.0 / sh[i][i] : 0.0;
+        T[kSlotRecip + i] = i < w ? 1.0 / sh[i][i] : 0.0;
+    // Inverses of the 4 x 4 diagonal blocks, one row per thread:
+    // D(i,i) = 1/u(i,i), D(i,j) = -(sum_{k=i}^{j-1} D(i,k) u(k,j)) / u(j,j).
+    for (int e = threadIdx.x; e < kMaxS; e += blockDim.x) {
+        const int b = e >> 2, i = e & 3, o = 4 * b;
+        double *D = T + kSlotDiag + 16 * b;
+        double d[4] = {0.0, 0.0, 0.0, 0.0};
+        if (o + i < w) {
+            d[i] = 1.0 / sh[o + i][o + i];
+            for (int j = i + 1; j < 4 && o + j < w; j++) {
+                double v = 0.0;
+                for (int k = i; k < j; k++) v = fma(d[k], sh[o + k][o + j], v);
+                d[j] = -v * (1.0 / sh[o + j][o + j]);
+            }
+        }
+        for (int j = 0; j < 4; j++) D[i + 4 * j] = d[j];
+    }
     const int info = info_s;
     __syncthreads();
     return info;
@@ -944,7 +960,7 @@ __global__ __launch_bounds__(256) void k_pass16(PassArgs a)
 
 constexpr int kB = 512;
 constexpr int kBTile = 10240;  // doubles of the LDS tile
-constexpr int kBPF = kBTile / kB;
+constexpr int kBPF = 16;       // prefetch registers per thread (ncols * rows <= 8192)
 
 int pass16b_rows(const PassArgs &a)
 {
@@ -1040,7 +1056,7 @@ __device__ __forceinline__ dbl4 zsteps_g(const double *sm, const double *__restr
 template <int MT, int NB, int F>
 __global__ __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_pass16b(PassArgs a)
 {
-    constexpr int S = 16, N1 = 32, TS = 32 * F, LD = TS + 1;
+    constexpr int N1 = 32, TS = 32 * F, LD = TS + 1;
     constexpr int LTS = F == 1 ? 5 : F == 2 ? 6 : F == 4 ? 7 : 8;
     extern __shared__ double sm[];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -1069,6 +1085,18 @@ __global__ __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(4, 4))) void
     }
     const int boff2 = (a.b0 + tb * 16 + li) * LD + lk;
     const bool doproj = proj && tb * 16 < a.nb && mb < Ma;
+    // Triangular solve operands of this wave's group (wave & 1), constant
+    // during the pass: B fragments (lane: row lk, column li) of the inverted
+    // diagonal blocks placed at columns 4b..4b+3, and of -T(4b+lk, li) for
+    // the columns right of block b.
+    const int gq = wave & 1;
+    const double *Tq = gq < a.ngrp ? ORTH_G(a, gq, T) : nullptr;
+    double bd[4], bt[3];
+#pragma unroll
+    for (int b = 0; b < 4; b++) {
+        bd[b] = (Tq && (li >> 2) == b) ? Tq[kSlotDiag + 16 * b + lk + 4 * (li & 3)] : 0.0;
+        if (b < 3) bt[b] = (Tq && li >= 4 * b + 4) ? -Tq[4 * b + lk + li * kLdT] : 0.0;
+    }
     dbl4 acc[MT];
 #pragma unroll
     for (int j = 0; j < MT; j++) acc[j] = dbl4{0.0, 0.0, 0.0, 0.0};
@@ -1088,7 +1116,7 @@ __global__ __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(4, 4))) void
         const double *q = px + r0;
 #pragma unroll
         for (int i = 0; i < kBPF; i++)
-            if (i < nld) pf[i] = ok ? q[i * pxs] : 0.0;
+            if (i < nld) pf[i] = ok ? __builtin_nontemporal_load(q + i * pxs) : 0.0;
     };
     if (rb < re) load(rb);
 
@@ -1098,7 +1126,6 @@ __global__ __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(4, 4))) void
         for (int i = 0; i < kBPF; i++)
             if (i < nld) sm[pl + i * PSTEP * LD] = pf[i];
         __syncthreads();
-        if (r0 + TS < re) load(r0 + TS);
 
         if (a.ngrp > 0) {
             const int nphase = a.seq ? a.ngrp : 1;
@@ -1181,38 +1208,51 @@ __global__ __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(4, 4))) void
                     }
                 }
                 __syncthreads();
-                // Row-wise forward substitution in the operation order of
-                // dtrsm_('R','U','N','N'): x_j = (x_j - sum_{k<j} x_k T_kj) / T_jj.
-                // One group per wave, so the factor and its reciprocal pivots
-                // are wave-uniform and come through the scalar cache; the row
-                // lives in registers.
-                {
-                    constexpr int NBLK = TS > 64 ? TS / 64 : 1;
-                    for (int task = wave; task < (ghi - glo) * NBLK; task += kB / 64) {
-                        const int g = __builtin_amdgcn_readfirstlane(glo + task / NBLK);
-                        const double *Tg = ORTH_G(a, g, T);
-                        // The factor is read-only during the pass: read it
-                        // through the constant address space (scalar loads).
-                        typedef const __attribute__((address_space(4))) double cdouble;
-                        cdouble *T = (cdouble *)Tg;
-                        const int row = (task % NBLK) * 64 + lane;
-                        if (Tg && row < TS) {
-                            cdouble *rv = T + kLdT * kLdT;
-                            const int o = ORTH_G(a, g, t0) * LD + row;
-                            double x[S];
+                // Triangular solve X <- X T^{-1} of each group, blocked by
+                // four columns on FP64 MFMA (the inverted-diagonal-block
+                // scheme of blocked dtrsm): x_b = b_b D_b, then the columns
+                // right of block b lose x_b T(b, >).  Wave w takes group
+                // (w & 1) and 16-row chunks (w >> 1) + 4 i; the chunk lives
+                // in MFMA accumulator layout (lane: column li, rows lk + 4 v)
+                // and each block goes through LDS once to become an A operand.
+                if (Tq && gq >= glo && gq < ghi) {
+                    const int t0 = ORTH_G(a, gq, t0), w = ORTH_G(a, gq, w);
+                    for (int ch = wave >> 1; ch < TS / 16; ch += 4) {
+                        const int base = (t0 + li) * LD + ch * 16 + lk;
+                        const int abase = t0 * LD + ch * 16 + li + lk * LD;
+                        dbl4 x;
 #pragma unroll
-                            for (int i = 0; i < S; i++) x[i] = sm[o + i * LD];
+                        for (int v = 0; v < 4; v++) x[v] = sm[base + 4 * v];
 #pragma unroll
-                            for (int j = 0; j < S; j++) {
-                                double v = x[j];
+                        for (int b = 0; b < 4; b++) {
+                            // Columns at or past the group width keep their
+                            // tile contents (they only feed each other).
+                            const bool mineb = (li >> 2) == b && li < w;
+                            if (mineb) {
 #pragma unroll
-                                for (int k = 0; k < j; k++) v = fma(-x[k], T[k + j * kLdT], v);
-                                x[j] = v * rv[j];
+                                for (int v = 0; v < 4; v++) sm[base + 4 * v] = x[v];
                             }
-                            const int w = ORTH_G(a, g, w);
+                            // Other lanes' LDS writes become visible to this
+                            // lane (the wave's LDS accesses execute in order);
+                            // the fence stops the compiler from forwarding the
+                            // lane's own earlier value instead of reloading.
+                            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+                            const double av = sm[abase + 4 * b * LD];
+                            if ((li >> 2) == b) x = dbl4{0.0, 0.0, 0.0, 0.0};
+                            x = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bd[b], x, 0, 0, 0);
+                            if (b < 3) {
+                                if (mineb) {
 #pragma unroll
-                            for (int i = 0; i < S; i++)
-                                if (i < w) sm[o + i * LD] = x[i];
+                                    for (int v = 0; v < 4; v++) sm[base + 4 * v] = x[v];
+                                }
+                                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+                                const double xv = sm[abase + 4 * b * LD];
+                                x = __builtin_amdgcn_mfma_f64_16x16x4f64(xv, bt[b], x, 0, 0, 0);
+                            }
+                        }
+                        if (li < w) {
+#pragma unroll
+                            for (int v = 0; v < 4; v++) sm[base + 4 * v] = x[v];
                         }
                     }
                 }
@@ -1222,10 +1262,14 @@ __global__ __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(4, 4))) void
                 for (int g = 0; g < a.ngrp; g++) {
                     const int t0 = ORTH_G(a, g, t0);
                     for (int cc = pcol; cc < ORTH_G(a, g, w); cc += PSTEP)
-                        a.X[r0 + prow + (int64_t)(t0 + cc) * a.ldX] = sm[(t0 + cc) * LD + prow];
+                        __builtin_nontemporal_store(sm[(t0 + cc) * LD + prow],
+                                                    a.X + r0 + prow + (int64_t)(t0 + cc) * a.ldX);
                 }
             }
         }
+        // Next tile's loads: issued here, after the substitution, so the
+        // prefetch registers are live only across the projection.
+        if (r0 + TS < re) load(r0 + TS);
 
         if (doproj) {
 #pragma unroll
@@ -1237,6 +1281,8 @@ __global__ __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(4, 4))) void
 #pragma unroll
                 for (int j = 0; j < MT; j++)
                     acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ac[j], bc, acc[j], 0, 0, 0);
+                // Bound the operand loads in flight (register pressure).
+                if ((ks & 1) == 1) __builtin_amdgcn_sched_barrier(0);
             }
         }
     }
@@ -1273,16 +1319,25 @@ hipError_t launch_pass16b_t(const PassArgs &a, size_t lds, int nwg, hipStream_t 
     return hipGetLastError();
 }
 
+// Only the (MT, F) pairs the plan can produce are instantiated: a tile of
+// TS = 32 F rows holds at most 8192 / TS columns, so MT <= 4 / F.
 template <int MT, int NB>
 hipError_t launch_pass16b_f(const PassArgs &a, int ts, size_t lds, int nwg, hipStream_t st)
 {
     switch (ts) {
     case 32: return launch_pass16b_t<MT, NB, 1>(a, lds, nwg, st);
-    case 64: return launch_pass16b_t<MT, NB, 2>(a, lds, nwg, st);
-    case 128: return launch_pass16b_t<MT, NB, 4>(a, lds, nwg, st);
-    case 256: return launch_pass16b_t<MT, NB, 8>(a, lds, nwg, st);
-    default: return hipErrorInvalidValue;
+    case 64:
+        if constexpr (MT <= 2) return launch_pass16b_t<MT, NB, 2>(a, lds, nwg, st);
+        break;
+    case 128:
+        if constexpr (MT == 1) return launch_pass16b_t<MT, NB, 4>(a, lds, nwg, st);
+        break;
+    case 256:
+        if constexpr (MT == 1) return launch_pass16b_t<MT, NB, 8>(a, lds, nwg, st);
+        break;
+    default: break;
     }
+    return hipErrorInvalidValue;
 }
 
 // Super-tile rows of the 512-thread s = 16 pass, or 0 when it does not apply.
