@@ -1018,6 +1018,17 @@ __device__ __forceinline__ dbl4 zsteps(const double *sm, int ao, int bo, int LD,
 // array (which would force a private copy of the arguments).
 #define ORTH_G(args_, idx_, fld_) ((idx_) == 0 ? (args_).g[0].fld_ : (args_).g[1].fld_)
 
+// Is Z tile column block nt (16 columns) consumed by some group?
+__device__ __forceinline__ bool ztile_used(const PassArgs &a, int nt)
+{
+    bool u = false;
+    for (int g = 0; g < a.ngrp; g++) {
+        const int z0 = ORTH_G(a, g, zoff), z1 = z0 + ORTH_G(a, g, w);
+        u = u || (nt * 16 < z1 && nt * 16 + 16 > z0);
+    }
+    return u;
+}
+
 // zsteps with the B operand (the coefficients) read from global memory.
 __device__ __forceinline__ dbl4 zsteps_g(const double *sm, const double *__restrict__ C, int ao,
                                          int bo, int LD, int N1, int k0, int k1e)
@@ -1137,7 +1148,8 @@ __global__ __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(4, 4))) void
                     // Z tile (wave & 3) over half (wave >> 2) of the k-steps;
                     // the halves are added in a fixed order through LDS.
                     const int item = wave & 3, mt = item & 1, nt = item >> 1, kh = wave >> 2;
-                    const bool mine = nks > 0 && (!a.seq || nt == (ORTH_G(a, p, zoff) >> 4));
+                    const bool mine = nks > 0 && (a.seq ? nt == (ORTH_G(a, p, zoff) >> 4)
+                                                        : ztile_used(a, nt));
                     dbl4 z = dbl4{0.0, 0.0, 0.0, 0.0};
                     if (mine) {
                         const int half = nks >> 1;
@@ -1166,7 +1178,8 @@ __global__ __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(4, 4))) void
                     // Z tiles (rows 16 ((wave >> 1) + 4 j), cols 16 (wave & 1)).
                     constexpr int NM = F > 1 ? F / 2 : 1;
                     const int nt = wave & 1, mt0 = wave >> 1;
-                    const bool mine = nks > 0 && (!a.seq || nt == (ORTH_G(a, p, zoff) >> 4));
+                    const bool mine = nks > 0 && (a.seq ? nt == (ORTH_G(a, p, zoff) >> 4)
+                                                        : ztile_used(a, nt));
                     dbl4 z[NM];
 #pragma unroll
                     for (int j = 0; j < NM; j++) z[j] = dbl4{0.0, 0.0, 0.0, 0.0};
@@ -1267,7 +1280,7 @@ __global__ __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(4, 4))) void
                 }
             }
         }
-        // Next tile's loads: issued here, after the substitution, so the
+        // Next tile's loads: issued after the triangular solve, so the
         // prefetch registers are live only across the projection.
         if (r0 + TS < re) load(r0 + TS);
 
