@@ -100,6 +100,116 @@ def pass_traffic(variant, m, n, s):
     return None
 
 
+def make_glued(torch, m, n, s, delta, seed):
+    """Blocks X_k = Q0 C_k + delta E_k (one fixed orthonormal m x s basis Q0),
+    as an n x m row-major tensor (= column-major m x n)."""
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    Q0 = torch.randn((s, m), generator=g, dtype=torch.float64, device="cuda")
+    for _ in range(2):
+        L = torch.linalg.cholesky(Q0 @ Q0.T)
+        Q0 = torch.linalg.inv(L) @ Q0
+    X = torch.empty((n, m), dtype=torch.float64, device="cuda")
+    for k in range(n // s):
+        C = torch.randn((s, s), generator=g, dtype=torch.float64, device="cuda")
+        X[k * s:(k + 1) * s] = C.T @ Q0
+        X[k * s:(k + 1) * s] += delta * torch.randn((s, m), generator=g, dtype=torch.float64,
+                                                     device="cuda")
+    del Q0
+    return X
+
+
+def run_one(torch, B, variant, X0, m, n, s, reps=3):
+    """Time `reps` factorizations of X0 (restored before each, outside the
+    events); returns (ms, status, loo, rel_residual) of the last one."""
+    X = torch.empty_like(X0)
+    R = torch.zeros(n * n, dtype=torch.float64, device="cuda")
+    st = torch.cuda.current_stream()
+    times = []
+    for k in range(reps + 1):
+        X.copy_(X0)
+        R.zero_()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        B.bcgs_device(variant, m, n, s, X.data_ptr(), m, R.data_ptr(), n, st.cuda_stream)
+        e1.record(st)
+        torch.cuda.synchronize()
+        if k:
+            times.append(e0.elapsed_time(e1))
+    status = B.status()
+    loo, res, xn = B.accuracy_device(m, n, X0.data_ptr(), m, X.data_ptr(), m, R.data_ptr(), n,
+                                     st.cuda_stream)
+    del X, R
+    return sorted(times)[len(times) // 2], status, loo, res / xn
+
+
+def suite(args):
+    """The measurement configurations beside the headline line: every variant
+    at m=2^24, n=256, s=16 for kappa in {1e2, 1e6, 1e10}; the reference's CPU
+    shape (m=1e5, n=200, s=10, kappa=1e8) on the GPU beside the CPU oracle on
+    the same input; and glued near-rank-deficient blocks at m=2^22, n=512,
+    s=32 for delta in {1e-6, 1e-10, 1e-13}.  One JSON object per case on
+    stdout (this mode prints no headline line)."""
+    import numpy as np
+    import torch
+
+    import bcgs_hip as B
+    import inputs
+    import oracle
+
+    B.lib()
+    variants = list(B.VARIANTS)
+    out = []
+
+    def emit(d):
+        out.append(d)
+        print(json.dumps(d), flush=True)
+
+    # Reference CPU shape: GPU and CPU oracle on identical bits.
+    m, n, s = 100000, 200, 10
+    X0 = make_input(torch, m, n, 1e8, seed=2)
+    Xh = X0.cpu().numpy().T
+    oracle.set_threads(args.cpu_threads)
+    for v in variants:
+        ms, st, loo, res = run_one(torch, B, v, X0, m, n, s)
+        t0 = time.perf_counter()
+        Qo, Ro, info = oracle.bcgs(v, Xh, s)
+        tc = time.perf_counter() - t0
+        emit({"config": "cpu-shape", "variant": v, "m": m, "n": n, "s": s, "kappa": 1e8,
+              "gpu_ms": round(ms, 3), "gpu_loo": loo, "gpu_rel_residual": res,
+              "gpu_info_block": st["info_block"], "cpu_ms": round(tc * 1e3, 1),
+              "cpu_threads": args.cpu_threads, "cpu_loo": inputs.loo(Qo),
+              "cpu_rel_residual": inputs.rel_residual(Xh, Qo, Ro), "cpu_info_block": info[0]})
+    del X0
+
+    # North-star shape, every variant, three conditionings.
+    m, n, s = 1 << 24, 256, 16
+    for kappa in (1e2, 1e6, 1e10):
+        X0 = make_input(torch, m, n, kappa, seed=1)
+        for v in variants:
+            ms, st, loo, res = run_one(torch, B, v, X0, m, n, s)
+            emit({"config": "fp64", "variant": v, "m": m, "n": n, "s": s, "kappa": kappa,
+                  "ms": round(ms, 3), "GBs_alg": round(b_alg(m, n, s) / (ms / 1e3) / 1e9, 1),
+                  "loo": loo, "rel_residual": res, "info_block": st["info_block"],
+                  "passes": st["passes"], "syncs": st["syncs"]})
+        del X0
+        torch.cuda.empty_cache()
+
+    # Glued stress.
+    m, n, s = 1 << 22, 512, 32
+    for delta in (1e-6, 1e-10, 1e-13):
+        X0 = make_glued(torch, m, n, s, delta, seed=3)
+        for v in variants:
+            ms, st, loo, res = run_one(torch, B, v, X0, m, n, s, reps=1)
+            emit({"config": "glued", "variant": v, "m": m, "n": n, "s": s, "delta": delta,
+                  "ms": round(ms, 3), "GBs_alg": round(b_alg(m, n, s) / (ms / 1e3) / 1e9, 1),
+                  "loo": loo, "rel_residual": res, "info_block": st["info_block"],
+                  "info_pivot": st["info_pivot"]})
+        del X0
+        torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -114,6 +224,9 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--suite", action="store_true",
+                    help="run the per-variant / per-kappa / glued measurement cases "
+                         "instead of the headline line (one JSON object per case)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="after timing, run one plain elementwise copy of X (read 8mn, "
                          "write 8mn bytes) to calibrate HBM byte counters")
@@ -124,6 +237,10 @@ def main():
               "shape; multi-GPU runs are not part of this build (--gpus 1).",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.suite:
+        suite(args)
+        return
 
     import torch
 

@@ -225,3 +225,37 @@ def test_fused_and_unfused_agree(variant, monkeypatch):
     assert stf["passes"] < stu["passes"]
     assert np.abs(Rf - Ru).max() <= 1e-11 * np.abs(Ru).max()
     assert np.abs(Qf - Qu).max() <= 1e-10
+
+
+@pytest.mark.parametrize("variant", VARIANTS)
+@pytest.mark.parametrize("shape", [(20000, 512, 32), (12000, 300, 16)])
+def test_wide_slabs(variant, shape):
+    """n beyond the fused pass's 272-column tile: the separate projection and
+    update kernels at the glued-stress width (n = 512, s = 32)."""
+    m, n, s = shape
+    X = I.svd_matrix(m, n, 1e3, seed=31)
+    check_against_oracle(variant, X, s)
+
+
+@pytest.mark.parametrize("env", ["ORTH_PASS256", "ORTH_GENERIC_PASS"])
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_alternate_pass_kernels(variant, env, monkeypatch):
+    """The 256-thread s = 16 pass and the generic pass agree with the oracle."""
+    monkeypatch.setenv("ORTH_PASS256", "1")
+    if env == "ORTH_GENERIC_PASS":
+        monkeypatch.setenv("ORTH_GENERIC_PASS", "1")
+    X = I.svd_matrix(6000, 80, 1e4, seed=32)
+    check_against_oracle(variant, X, 16)
+
+
+@pytest.mark.parametrize("variant", ["bcgsi2P1s", "bcgsi2", "bcgsi2P2s", "bcgspipi2"])
+def test_glued_wide(variant):
+    """Glued blocks at n = 512, s = 32: breakdown reported on the same block
+    as the oracle; otherwise LOO at the oracle's level."""
+    X = I.glued_matrix(16384, 512, 32, 1e-10, seed=33)
+    say("case glued wide", variant)
+    Qo, Ro, info_o = O.bcgs(variant, X, 32)
+    Qg, Rg, st = B.bcgs(variant, X, 32)
+    assert st["info_block"] == info_o[0]
+    if info_o == (0, 0):
+        assert within10(I.loo(Qg), I.loo(Qo), 1e-13)
