@@ -35,9 +35,27 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def finite(v):
+    """JSON-safe copy: NaN / inf (a reported breakdown) become null."""
+    if isinstance(v, float):
+        return v if v == v and abs(v) != float("inf") else None
+    if isinstance(v, dict):
+        return {k: finite(x) for k, x in v.items()}
+    if isinstance(v, list):
+        return [finite(x) for x in v]
+    return v
+
+
 def b_alg(m, n, s, elem=8):
     N = 1 + (n - s + s - 1) // s
     return elem * m * (2 * n + s * N * (N - 1) // 2)
+
+
+def b_alg_mixed(m, n, s):
+    """Algorithmic bytes of the mixed form: read X once in FP32, write Q once
+    and re-read the Q prefix once per block in FP64."""
+    N = 1 + (n - s + s - 1) // s
+    return 4 * m * n + 8 * m * (n + s * N * (N - 1) // 2)
 
 
 def f_alg(m, n):
@@ -163,7 +181,7 @@ def suite(args):
 
     def emit(d):
         out.append(d)
-        print(json.dumps(d), flush=True)
+        print(json.dumps(finite(d)), flush=True)
 
     # Reference CPU shape: GPU and CPU oracle on identical bits.
     m, n, s = 100000, 200, 10
@@ -224,6 +242,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true",
+                    help="skip the mixed-precision leg (FP32-stored input)")
     ap.add_argument("--suite", action="store_true",
                     help="run the per-variant / per-kappa / glued measurement cases "
                          "instead of the headline line (one JSON object per case)")
@@ -295,6 +315,39 @@ def main():
         acc = {"loo": loo, "rel_residual": res / xn}
         log("LOO %.3e  residual %.3e" % (loo, res / xn))
 
+    # Mixed-precision leg: the same input rounded to FP32 storage; first-pass
+    # products in each reduced precision, FP64 reorthogonalization.
+    mixed = {}
+    if not args.no_mixed and args.variant == "bcgsi2P1s" and s == 16 and n <= 256:
+        X32 = X0.float()
+        base_loo = acc.get("loo")
+        for prec in ("fp64", "bf16", "fp16", "bf16x3"):
+            ts = []
+            for k in range(1 + min(args.steps, 3)):
+                R.zero_()
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                B.bcgs_mixed_device(args.variant, prec, m, n, s, X32.data_ptr(), m, X.data_ptr(),
+                                    m, R.data_ptr(), n, sp)
+                e1.record(stream)
+                torch.cuda.synchronize()
+                if k:
+                    ts.append(e0.elapsed_time(e1))
+            mms = sum(ts) / len(ts)
+            stm = B.status()
+            d = {"ms": round(mms, 3),
+                 "GBs_alg": round(b_alg_mixed(m, n, s) / (mms / 1e3) / 1e9, 1),
+                 "speedup_vs_fp64": round(ms / mms, 3), "info_block": stm["info_block"]}
+            if not args.no_accuracy:
+                lo, re_, xn = B.accuracy_device(m, n, X0.data_ptr(), m, X.data_ptr(), m,
+                                                R.data_ptr(), n, sp)
+                d.update({"loo": lo, "rel_residual": re_ / xn,
+                          "equal_loo": bool(base_loo is not None and lo <= 10 * max(base_loo, 1e-14))})
+            mixed[prec] = d
+            log("mixed %s: %s" % (prec, json.dumps(finite(d))))
+        del X32
+
     # Dominant kernel class and its roofline.
     dom = max(kt.items(), key=lambda kv: kv[1]["ms"])
     name, d = dom
@@ -339,11 +392,13 @@ def main():
         "roofline": roof,
     }
     out.update(acc)
+    if mixed:
+        out["mixed"] = mixed
     if not args.no_cpu:
         log("CPU baseline ...")
         out["cpu_baseline"] = cpu_baseline(X0, args.variant, n, s, min(args.cpu_rows, m),
                                            args.cpu_threads, m)
-    print(json.dumps(out), flush=True)
+    print(json.dumps(finite(out)), flush=True)
 
 
 if __name__ == "__main__":

@@ -1,7 +1,7 @@
 /*
  * Extended interface of liborth_hip.so: device-pointer and stream entry
- * points, status reporting, the accuracy harness, the mixed-precision form
- * and the batched form.  The five reference-compatible symbols are in
+ * points, status reporting, the accuracy harness and the mixed-precision
+ * form.  The five reference-compatible symbols are in
  * orth.h; nothing here has a counterpart in the reference's include/orth.h
  * unless its comment cites one.
  *
@@ -29,12 +29,12 @@ enum {
     ORTH_P2S = 4   /* bcgsi2P2s  */
 };
 
-/* Precision of the bulk products in the mixed form. */
+/* Precision of the first-pass products in the mixed form. */
 enum {
-    ORTH_PREC_FP64 = 0,   /* everything FP64 (the drop-in arithmetic)        */
-    ORTH_PREC_BF16 = 1,   /* Q^T X and Q S on bf16 MFMA, FP32 accumulation    */
-    ORTH_PREC_FP16 = 2,   /* same with fp16 operands                          */
-    ORTH_PREC_BF16X3 = 3  /* split bf16x3 operands (hi*hi + hi*lo + lo*hi)   */
+    ORTH_PREC_FP64 = 0,   /* FP64 products; only the input storage is FP32    */
+    ORTH_PREC_BF16 = 1,   /* bf16 MFMA operands, FP32 accumulation            */
+    ORTH_PREC_FP16 = 2,   /* fp16 MFMA operands, FP32 accumulation            */
+    ORTH_PREC_BF16X3 = 3  /* split bf16 operands (hi*lo + lo*hi + hi*hi)      */
 };
 
 typedef struct {
@@ -54,6 +54,24 @@ int orth_last_status(orth_status_t *st);
  * invalid shape / HIP error. */
 int orth_bcgs_device(int variant, int64_t m, int64_t n, int64_t s, double *dX,
                      int64_t ldX, double *dR, int64_t ldR, void *stream);
+
+/* Mixed-precision form of BCGSI+P-1S (no counterpart in the reference):
+ * the input X32 (m x n FP32, ld ldX32, device, not modified) is
+ * orthogonalized into Q (m x n FP64, ld ldQ, device) and R (FP64, as in
+ * orth.h).  `precision` selects the arithmetic of the first-pass products
+ * (the update X - Q S and the projection Q^T X with the Gram X^T X of a
+ * block): ORTH_PREC_BF16 / _FP16 on MFMA with FP32 accumulation per 32-row
+ * tile, ORTH_PREC_BF16X3 (FP32 values split into two bf16 terms, three
+ * products), or ORTH_PREC_FP64.  The reorthogonalization (second-pass
+ * update and projection), the Pythagorean subtraction, the Cholesky factors
+ * and the triangular solves are FP64 in every case.  Supports variant
+ * ORTH_P1S with s = 16 and n <= 256.  Returns 0, or 2 for an unsupported
+ * shape / precision or a HIP error; breakdown is reported by
+ * orth_last_status() as for the FP64 form. */
+int orth_bcgs_mixed_device(int variant, int precision, int64_t m, int64_t n,
+                           int64_t s, const float *dX32, int64_t ldX32,
+                           double *dQ, int64_t ldQ, double *dR, int64_t ldR,
+                           void *stream);
 
 /* Accuracy harness of the reference (SRC/util.c:36-75) on device data:
  * out[0] = ||Q^T Q - I||_F, out[1] = ||Q R - X||_F, out[2] = ||X||_F.

@@ -15,8 +15,8 @@ CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "liborth_hip.so")
 
-SOURCES = ["orth_kernels.hip", "orth_host.cpp"]
-HEADERS = ["orth_kernels.h"]
+SOURCES = ["orth_kernels.hip", "orth_mixed.hip", "orth_host.cpp"]
+HEADERS = ["orth_kernels.h", "orth_device.h"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

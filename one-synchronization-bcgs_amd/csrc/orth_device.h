@@ -1,0 +1,62 @@
+// Device helpers shared by the kernel translation units of liborth_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "orth_kernels.h"
+
+namespace orth {
+
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+// Group fields by a runtime index without indexing the kernel-argument
+// array (which would force a private copy of the arguments).
+#define ORTH_G(args_, idx_, fld_) ((idx_) == 0 ? (args_).g[0].fld_ : (args_).g[1].fld_)
+
+// Is Z tile column block nt (16 columns) consumed by some group?
+__device__ __forceinline__ bool ztile_used(const PassArgs &a, int nt)
+{
+    bool u = false;
+    for (int g = 0; g < a.ngrp; g++) {
+        const int z0 = ORTH_G(a, g, zoff), z1 = z0 + ORTH_G(a, g, w);
+        u = u || (nt * 16 < z1 && nt * 16 + 16 > z0);
+    }
+    return u;
+}
+
+// zsteps with the B operand (the coefficients) read from global memory.
+__device__ __forceinline__ dbl4 zsteps_g(const double *sm, const double *__restrict__ C, int ao,
+                                         int bo, int LD, int N1, int k0, int k1e)
+{
+    dbl4 z = dbl4{0.0, 0.0, 0.0, 0.0};
+    int ks = k0;
+    if (k1e - k0 >= 4) {
+        double b0 = C[bo + 4 * ks * N1], b1 = C[bo + 4 * (ks + 1) * N1],
+               b2 = C[bo + 4 * (ks + 2) * N1], b3 = C[bo + 4 * (ks + 3) * N1];
+        double a0 = sm[ao + 4 * ks * LD], a1 = sm[ao + 4 * (ks + 1) * LD],
+               a2 = sm[ao + 4 * (ks + 2) * LD], a3 = sm[ao + 4 * (ks + 3) * LD];
+        for (; ks + 4 <= k1e; ks += 4) {
+            const double c0 = a0, c1 = a1, c2 = a2, c3 = a3;
+            const double d0 = b0, d1 = b1, d2 = b2, d3 = b3;
+            const int nx = min(ks + 4, k1e - 4);
+            b0 = C[bo + 4 * nx * N1];
+            b1 = C[bo + 4 * (nx + 1) * N1];
+            b2 = C[bo + 4 * (nx + 2) * N1];
+            b3 = C[bo + 4 * (nx + 3) * N1];
+            a0 = sm[ao + 4 * nx * LD];
+            a1 = sm[ao + 4 * (nx + 1) * LD];
+            a2 = sm[ao + 4 * (nx + 2) * LD];
+            a3 = sm[ao + 4 * (nx + 3) * LD];
+            z = __builtin_amdgcn_mfma_f64_16x16x4f64(c0, d0, z, 0, 0, 0);
+            z = __builtin_amdgcn_mfma_f64_16x16x4f64(c1, d1, z, 0, 0, 0);
+            z = __builtin_amdgcn_mfma_f64_16x16x4f64(c2, d2, z, 0, 0, 0);
+            z = __builtin_amdgcn_mfma_f64_16x16x4f64(c3, d3, z, 0, 0, 0);
+        }
+    }
+    for (; ks < k1e; ks++)
+        z = __builtin_amdgcn_mfma_f64_16x16x4f64(sm[ao + 4 * ks * LD], C[bo + 4 * ks * N1], z, 0,
+                                                 0, 0);
+    return z;
+}
+
+}  // namespace orth

@@ -453,7 +453,7 @@ class Run {
         PassGroup g[2];
     } pend_;
 
-    void pend_group(int64_t t0, int64_t w, int zoff, int64_t k, const double *T)
+    void pend_group(int64_t t0, int64_t w, int zoff, int64_t k, const double *T, bool lp = false)
     {
         PassGroup &g = pend_.g[pend_.ngrp++];
         g.t0 = (int)t0;
@@ -461,7 +461,18 @@ class Run {
         g.zoff = zoff;
         g.k = (int)k;
         g.T = T;
+        g.lp = lp ? 1 : 0;
     }
+
+    // Mixed form: the input comes from FP32 storage and the marked
+    // first-pass products run in the reduced precision mx (kMx*).
+    void set_mixed(int mx, const float *x32, int64_t ldx32)
+    {
+        mx_ = mx;
+        x32_ = x32;
+        ldx32_ = ldx32;
+    }
+    bool lp_first_pass() const { return mx_ != 0 && mx_ != kMxFP64; }
 
     // One fused pass: the pending update, then P <- A^T B (na == 0: none).
     double *pass(int64_t a0, int64_t na, int64_t b0, int64_t nb)
@@ -508,9 +519,18 @@ class Run {
         // 256-thread passes (whose s = 16 form needs the tile padded to
         // ceil(Ma / 4) projection tiles per wave).
         int mt = 0, nbt = 0;
-        const int ts16 = (S_ == 16 && !getenv("ORTH_PASS256")) ? pass16b_plan(pa, &mt, &nbt) : 0;
+        const int ts16 =
+            (S_ == 16 && !mx_ && !getenv("ORTH_PASS256")) ? pass16b_plan(pa, &mt, &nbt) : 0;
         int64_t unit = kPassRows;
-        if (ts16 > 0) {
+        if (mx_) {
+            // Columns at or past the written frontier still live only in X32.
+            pa.X32 = x32_ + cb * ldx32_;
+            pa.ldX32 = ldx32_;
+            pa.raw0 = (int)std::max<int64_t>(0, frontier_ - cb);
+            if (pa.raw0 < pa.ncols && pa.raw0 % 16 != 0) return nullptr;
+            pa.lpb = lp_from_ >= 0 ? (int)(lp_from_ - cb) : (1 << 30);
+            if (pass16m_lds_bytes(pa) == 0) return nullptr;
+        } else if (ts16 > 0) {
             unit = ts16;
         } else if (S_ == 16 && na > 0) {
             const int64_t mtp = ((na + 15) / 16 + 3) / 4;
@@ -518,7 +538,10 @@ class Run {
         }
         // Two 512-thread workgroups per CU when the s = 16 pass applies (its
         // LDS is under half a CU's), so one hides the other's barriers.
-        int nwg = std::max(1, c_.ncu) * (ts16 > 0 ? 2 : 1);
+        // The mixed pass with reduced-precision products runs one 512-thread
+        // workgroup per CU (its operand conversions need two waves' worth of
+        // registers); with FP64 products it runs two, like the FP64 pass.
+        int nwg = std::max(1, c_.ncu) * (ts16 > 0 || mx_ == kMxFP64 ? 2 : 1);
         int64_t rows = (m_ + nwg - 1) / nwg;
         rows = (rows + unit - 1) / unit * unit;
         pa.rows_per = rows;
@@ -531,9 +554,13 @@ class Run {
             wcols += pend_.g[g].w;
         }
         if (!timed(c_, ORTH_K_PASS, st_, flops, 8.0 * m_ * (ncols + wcols), [&] {
+                if (mx_) return launch_pass16m(pa, mx_, nwg, st_);
                 return ts16 > 0 ? launch_pass16b(pa, nwg, st_) : launch_pass(S_, pa, nwg, st_);
             }))
             return nullptr;
+        for (int g = 0; g < pend_.ngrp; g++)
+            frontier_ = std::max<int64_t>(frontier_, pend_.g[g].t0 + pend_.g[g].w);
+        lp_from_ = -1;
         pend_ = Pending();
         passes_++;
         if (na == 0) return c_.P[pcur_];
@@ -590,7 +617,7 @@ class Run {
         pend_.k1 = (int)(c + wb);
         pend_.seq = seq ? 1 : 0;
         pend_group(c, wb, 0, c, Om_);
-        if (wn > 0) pend_group(c + wb, wn, S_, c + wb, next_T ? Sig_ : nullptr);
+        if (wn > 0) pend_group(c + wb, wn, S_, c + wb, next_T ? Sig_ : nullptr, lp_first_pass());
     }
 
     bool p1s_f()
@@ -598,14 +625,18 @@ class Run {
         if (!intra_qr_f(0, 0, s_, Rd_, ldR_)) return false;
         if (N_ == 1) return flush();
         const int64_t w1 = bw(1);
+        // Mixed form: the projection of a block's raw columns (first pass)
+        // and its first-pass update run in the reduced precision.
+        if (lp_first_pass()) lp_from_ = s_;
         double *P = pass(0, s_ + w1, s_, w1);
         if (!P || !small(sa(OP_OPEN_PYTH, 0, 1, s_, w1, 0, P, s_ + w1))) return false;
         pend_.k1 = (int)s_;
-        pend_group(s_, w1, 0, s_, Sig_);
+        pend_group(s_, w1, 0, s_, Sig_, lp_first_pass());
         int64_t c = s_;
         for (int64_t b = 1; b < N_; b++) {
             const int64_t wb = bw(b), wn = (b + 1 < N_) ? bw(b + 1) : 0;
             const int64_t na = c + wb + wn;
+            if (lp_first_pass() && wn > 0) lp_from_ = c + wb;
             P = pass(0, na, c, wb + wn);
             if (!P || !small(sa(OP_CLOSE_OPEN, F_PYTH_NEXT, b, c, wb, wn, P, na))) return false;
             pend_close_open(c, wb, wn, true, false);
@@ -702,6 +733,10 @@ class Run {
     bool run(int variant)
     {
         fused_ = !getenv("ORTH_UNFUSED") && fused_fits();
+        if (mx_) {
+            if (!fused_ || S_ != 16 || variant != ORTH_P1S) return false;
+            return p1s_f();
+        }
         if (fused_) {
             switch (variant) {
             case ORTH_P1S: return p1s_f();
@@ -742,6 +777,11 @@ class Run {
     bool debug_ = getenv("ORTH_DEBUG_DUMP") != nullptr;
     bool fused_ = false;
     int n1_ = 32;
+    int mx_ = 0;                // mixed form: reduced precision (kMx*), 0 = plain FP64
+    const float *x32_ = nullptr;
+    int64_t ldx32_ = 0;
+    int64_t frontier_ = 0;      // columns [0, frontier_) have been written to X
+    int64_t lp_from_ = -1;      // next pass: B columns from here on are reduced precision
 
     // Debug aid (ORTH_DEBUG_DUMP set): print the reduced projection and the
     // factors after each small step.
@@ -806,10 +846,12 @@ bool shape_ok(int64_t m, int64_t n, int64_t s, int64_t ldX, int64_t ldR)
 // Enqueue one variant on device data.  Rdev_out receives R (device or host
 // memory, per kind).  Returns 0 or 2.
 int run_device(DevCtx &c, int variant, int64_t m, int64_t n, int64_t s, double *dX,
-               int64_t ldX, double *Rout, int64_t ldR, hipMemcpyKind rkind, hipStream_t st)
+               int64_t ldX, double *Rout, int64_t ldR, hipMemcpyKind rkind, hipStream_t st,
+               int mx = 0, const float *x32 = nullptr, int64_t ldx32 = 0)
 {
     if (c.done_valid && !ok(hipStreamWaitEvent(st, c.done, 0))) return 2;
     Run r(c, m, n, s, dX, ldX, st);
+    if (mx) r.set_mixed(mx, x32, ldx32);
     if (!r.prepare()) return 2;
     if (!r.run(variant)) return 2;
     if (!r.copy_R_out(Rout, ldR, rkind)) return 2;
@@ -920,6 +962,29 @@ int orth_last_status(orth_status_t *st)
     fill_status(c);
     *st = c.last;
     return 0;
+}
+
+int orth_bcgs_mixed_device(int variant, int precision, int64_t m, int64_t n, int64_t s,
+                           const float *dX32, int64_t ldX32, double *dQ, int64_t ldQ, double *dR,
+                           int64_t ldR, void *stream)
+{
+    if (!dX32 || !dQ || !dR || !shape_ok(m, n, s, ldQ, ldR) || ldX32 < m) return 2;
+    if (variant != ORTH_P1S || s != 16 || n > 256) return 2;
+    int mx = 0;
+    switch (precision) {
+    case ORTH_PREC_FP64: mx = kMxFP64; break;
+    case ORTH_PREC_BF16: mx = kMxBF16; break;
+    case ORTH_PREC_FP16: mx = kMxFP16; break;
+    case ORTH_PREC_BF16X3: mx = kMxBF16X3; break;
+    default: return 2;
+    }
+    tl_hip_error = 0;
+    const int dev = current_device();
+    DevCtx &c = g_ctx[dev];
+    std::lock_guard<std::mutex> lk(c.mu);
+    if (!ctx_init(c)) return 2;
+    return run_device(c, variant, m, n, s, dQ, ldQ, dR, ldR, hipMemcpyDeviceToDevice,
+                      (hipStream_t)stream, mx, dX32, ldX32);
 }
 
 int orth_bcgs_device(int variant, int64_t m, int64_t n, int64_t s, double *dX, int64_t ldX,

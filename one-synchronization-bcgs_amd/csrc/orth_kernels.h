@@ -116,6 +116,7 @@ struct PassGroup {
     int zoff;        // first column of Z subtracted from it
     int k;           // inner dimension of its product in sequential mode
     const double *T; // w x w upper factor (ld kLdT) or nullptr
+    int lp;          // mixed form: its product runs in the reduced precision
 };
 
 struct PassArgs {
@@ -133,6 +134,13 @@ struct PassArgs {
     int a0, na, b0, nb;  // step 2 ranges (na == 0: no projection)
     double *part;
     int64_t rows_per;
+    // Mixed form only: columns >= raw0 (never written yet) are read from the
+    // FP32 input X32 instead of X; projection tiles whose B columns start at
+    // or after lpb run in the reduced precision.
+    const float *X32;
+    int64_t ldX32;
+    int raw0;
+    int lpb;
 };
 
 constexpr int kPassRows = 32;
@@ -147,6 +155,14 @@ hipError_t launch_pass(int S, const PassArgs &a, int nwg, hipStream_t st);
 // multiple of it.
 int pass16b_plan(const PassArgs &a, int *mt, int *nb);
 hipError_t launch_pass16b(const PassArgs &a, int nwg, hipStream_t st);
+
+// Mixed-precision s = 16 pass (32-row tiles, 512 threads): mx selects the
+// reduced precision of the marked products (1 bf16, 2 fp16, 3 bf16x3 split,
+// 4 none: FP64 arithmetic on FP32-stored input).  pass16m_lds_bytes returns
+// 0 when the shape does not fit it.
+enum { kMxBF16 = 1, kMxFP16 = 2, kMxBF16X3 = 3, kMxFP64 = 4 };
+size_t pass16m_lds_bytes(const PassArgs &a);
+hipError_t launch_pass16m(const PassArgs &a, int mx, int nwg, hipStream_t st);
 
 // Zero n entries with a kernel (no memset nodes, so the sequence can be
 // captured in a graph).

@@ -75,3 +75,20 @@ def test_no_cpu_fallback_when_library_missing(tmp_path):
     out = subprocess.run([sys.executable, "-c", code], capture_output=True,
                          text=True).stdout
     assert out.startswith("raised")
+
+
+@pytest.mark.parametrize("args", [
+    ("bcgsi2", "bf16", 4096, 32, 16),      # only BCGSI+P-1S has a mixed form
+    ("bcgsi2P1s", "bf16", 4096, 32, 8),    # block width 16 only
+    ("bcgsi2P1s", "bf16", 4096, 272, 16),  # n <= 256
+])
+def test_mixed_rejects_unsupported_without_device(args):
+    v, prec, m, n, s = args
+    L = B.lib()
+    rc = L.orth_bcgs_mixed_device(B.VARIANTS[v], B.PRECISIONS[prec], m, n, s,
+                                  ctypes.c_void_p(16), m, ctypes.c_void_p(16), m,
+                                  ctypes.c_void_p(16), n, None)
+    assert rc == 2
+    rc = L.orth_bcgs_mixed_device(0, 7, m, 32, 16, ctypes.c_void_p(16), m,
+                                  ctypes.c_void_p(16), m, ctypes.c_void_p(16), 32, None)
+    assert rc == 2

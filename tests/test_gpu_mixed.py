@@ -1,0 +1,87 @@
+"""GPU checks of the mixed-precision form (orth_bcgs_mixed_device).
+
+* FP64 products on FP32-stored input: same schedule and arithmetic as the
+  FP64 form, so Q and R match the CPU oracle run on the FP32-rounded input to
+  the FP64 parity tolerance (1e-11 relative in R).
+* bf16 / fp16 / bf16x3 first-pass products: the FP64 second pass restores
+  orthogonality, so at kappa = 1e2 the loss of orthogonality stays within 10x
+  of the oracle's FP64 value; the first-pass update error stays in the
+  residual, which must sit at the level of each format's unit roundoff
+  (within [0.1x, 10x] of 2^-9 for bf16, 2^-12 for fp16, 2^-18 for bf16x3,
+  scaled by the measured constant 1.0), which also shows that the reduced
+  precision is really used.
+"""
+import numpy as np
+import pytest
+
+import bcgs_hip as B
+import inputs as I
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def mixed(X, s, precision):
+    """Run the mixed form on X rounded to FP32; returns (Q, R, status)."""
+    m, n = X.shape
+    X32 = np.asfortranarray(X.astype(np.float32))
+    x32 = torch.from_numpy(np.ascontiguousarray(X32.T)).cuda()
+    q = torch.zeros((n, m), dtype=torch.float64, device="cuda")
+    r = torch.zeros(n * n, dtype=torch.float64, device="cuda")
+    B.bcgs_mixed_device("bcgsi2P1s", precision, m, n, s, x32.data_ptr(), m, q.data_ptr(), m,
+                        r.data_ptr(), n)
+    B.sync()
+    torch.cuda.synchronize()
+    st = B.status()
+    Q = q.cpu().numpy().T
+    R = r.cpu().numpy().reshape((n, n), order="F")
+    return Q, R, st, X32.astype(np.float64)
+
+
+@pytest.mark.parametrize("shape", [(20000, 128), (8192, 64), (5000, 80)])
+def test_fp64_products_match_oracle(shape):
+    m, n = shape
+    X = I.svd_matrix(m, n, 1e4, seed=41)
+    say("case mixed fp64", shape)
+    Q, R, st, Xr = mixed(X, 16, "fp64")
+    Qo, Ro, info = O.bcgs("bcgsi2P1s", Xr, 16)
+    assert st["hip_error"] == 0 and st["info_block"] == 0 and info == (0, 0)
+    assert np.abs(R - Ro).max() <= 1e-11 * np.abs(Ro).max()
+    assert np.abs(Q - Qo).max() <= 1e-10
+    assert np.all(np.tril(R, -1) == 0)
+
+
+LEVEL = {"bf16": 2.0 ** -9, "fp16": 2.0 ** -12, "bf16x3": 2.0 ** -18}
+
+
+@pytest.mark.parametrize("precision", ["bf16", "fp16", "bf16x3"])
+def test_reduced_precision_first_pass(precision):
+    m, n = 65536, 128
+    X = I.svd_matrix(m, n, 1e2, seed=42)
+    say("case mixed", precision)
+    Q, R, st, Xr = mixed(X, 16, precision)
+    assert st["hip_error"] == 0 and st["info_block"] == 0
+    Qo, Ro, _ = O.bcgs("bcgsi2P1s", Xr, 16)
+    loo, loo_o = I.loo(Q), I.loo(Qo)
+    assert loo <= 10 * max(loo_o, 1e-14), (loo, loo_o)
+    res = I.rel_residual(Xr, Q, R)
+    lvl = LEVEL[precision]
+    assert 0.1 * lvl <= res <= 10 * lvl, (res, lvl)
+    # R's upper triangle stays close to the FP64 factorization at that level.
+    assert np.abs(R - Ro).max() <= 100 * lvl * np.abs(Ro).max()
+
+
+def test_mixed_rejects_unsupported():
+    x = torch.zeros((32, 4096), dtype=torch.float32, device="cuda")
+    q = torch.zeros((32, 4096), dtype=torch.float64, device="cuda")
+    r = torch.zeros(32 * 32, dtype=torch.float64, device="cuda")
+    for v, s in [("bcgsi2", 16), ("bcgsi2P1s", 8)]:
+        with pytest.raises(RuntimeError):
+            B.bcgs_mixed_device(v, "bf16", 4096, 32, s, x.data_ptr(), 4096, q.data_ptr(), 4096,
+                                r.data_ptr(), 32)
