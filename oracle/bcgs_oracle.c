@@ -63,12 +63,20 @@ void oracle_plant(int mistake) { g_plant = mistake; }
  * longer rounding-level, so the tests can pin them.  It also models a
  * first pass computed in reduced precision. */
 static double g_noise = 0.0;
+static int g_noise_abs = 0;
 void oracle_set_first_pass_noise(double eta) { g_noise = eta; }
+/* abs != 0: the perturbation of entry (i, j) is eta * ||S(:, j)||_2 * r
+ * (the error model of a dot product in reduced precision) instead of
+ * eta * |S(i, j)| * r. */
+void oracle_set_noise_mode(int abs_mode) { g_noise_abs = abs_mode; }
 
 static void perturb(idx rows, idx cols, double *S, idx ld, idx blk)
 {
     if (g_noise == 0.0) return;
-    for (idx j = 0; j < cols; j++)
+    for (idx j = 0; j < cols; j++) {
+        double cn = 0.0;
+        for (idx i = 0; i < rows; i++) cn += S[i + j * ld] * S[i + j * ld];
+        cn = sqrt(cn);
         for (idx i = 0; i < rows; i++) {
             uint64_t z = (uint64_t)(i + 1) * 0x9E3779B97F4A7C15ull ^
                          (uint64_t)(j + 7) * 0xBF58476D1CE4E5B9ull ^
@@ -77,8 +85,12 @@ static void perturb(idx rows, idx cols, double *S, idx ld, idx blk)
             z *= 0xD6E8FEB86648D5A5ull;
             z ^= z >> 29;
             const double r = (double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;
-            S[i + j * ld] *= 1.0 + g_noise * r;
+            if (g_noise_abs)
+                S[i + j * ld] += g_noise * cn * r;
+            else
+                S[i + j * ld] *= 1.0 + g_noise * r;
         }
+    }
 }
 
 void oracle_set_threads(int nthreads) { g_threads = nthreads < 1 ? 1 : nthreads; }

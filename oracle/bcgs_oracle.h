@@ -73,6 +73,9 @@ void oracle_plant(int mistake);
 
 /* Relative perturbation of every first-pass projection (tests only). */
 void oracle_set_first_pass_noise(double eta);
+/* 1: perturbation proportional to the column norm of the projection (a
+ * reduced-precision dot product); 0: relative to each entry. */
+void oracle_set_noise_mode(int abs_mode);
 
 /* Primitives, exposed so the tests can pin them against numpy. */
 void oracle_gram(int64_t m, int64_t na, const double *A, int64_t lda,

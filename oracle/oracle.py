@@ -64,6 +64,8 @@ def lib():
                                     _dp, _i64, _dp, _i64]
         L.oracle_set_first_pass_noise.restype = None
         L.oracle_set_first_pass_noise.argtypes = [ctypes.c_double]
+        L.oracle_set_noise_mode.restype = None
+        L.oracle_set_noise_mode.argtypes = [ctypes.c_int]
         L.oracle_plant.restype = None
         L.oracle_plant.argtypes = [ctypes.c_int]
         L.oracle_set_threads.restype = None
@@ -85,8 +87,11 @@ def plant(mistake):
     lib().oracle_plant(int(mistake))
 
 
-def set_first_pass_noise(eta):
-    """Perturb every first-pass projection by a relative eta (0 removes it)."""
+def set_first_pass_noise(eta, column_scaled=False):
+    """Perturb every first-pass projection by eta (0 removes it): relative to
+    each entry, or (column_scaled) relative to the column norm, the error of
+    a dot product computed in reduced precision."""
+    lib().oracle_set_noise_mode(1 if column_scaled else 0)
     lib().oracle_set_first_pass_noise(float(eta))
 
 

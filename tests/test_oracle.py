@@ -308,3 +308,24 @@ def test_planted_mistakes_are_caught(mistake):
     finally:
         O.plant(0)
         O.set_first_pass_noise(0.0)
+
+
+def test_reduced_precision_first_pass_limit():
+    """Column-scaled error eta in the first-pass projections (a dot product
+    in reduced precision): the second pass removes it while eta kappa^2 < 1,
+    but the one-sync Pythagorean Cholesky chol(X^T X - S^T S) of BCGSI+P-1S
+    breaks down once eta kappa^2 >> 1, whereas BCGSI+ (intra-block QR of the
+    computed first-pass vectors) keeps O(u) orthogonality.  This bounds the
+    conditioning at which a low-precision first pass can match FP64."""
+    X = I.svd_matrix(8000, 64, 1e4, seed=44)
+    try:
+        O.set_first_pass_noise(1e-10, column_scaled=True)  # eta kappa^2 = 1e-2
+        Q, R, info = O.bcgs("bcgsi2P1s", X, 16)
+        assert info == (0, 0) and I.loo(Q) < 1e-13
+        O.set_first_pass_noise(1e-5, column_scaled=True)   # eta kappa^2 = 1e3
+        Q, R, info = O.bcgs("bcgsi2P1s", X, 16)
+        assert info != (0, 0) or not I.loo(Q) < 1e-10
+        Q, R, info = O.bcgs("bcgsi2", X, 16)                # eta kappa = 0.1
+        assert info == (0, 0) and I.loo(Q) < 1e-13
+    finally:
+        O.set_first_pass_noise(0.0)
