@@ -228,6 +228,104 @@ def suite(args):
     return out
 
 
+def batched(args):
+    """Batched form: `--batch-count` independent orthogonalizations of uneven
+    shapes (m log-uniform in [4096, 2^20], n in {32, 64, 128}, s in
+    {4, 8, 16}, kappa log-uniform in [1e2, 1e10]) in one graph-captured
+    launch after one tuning call.  One JSON object per variant / precision:
+    aggregate algorithmic GB/s, ms, tuned configuration, breakdown count and
+    per-instance LOO statistics (instances without breakdown), plus the CPU
+    oracle's LOO on the smallest instances for comparison."""
+    import numpy as np
+    import torch
+
+    import bcgs_hip as B
+    import inputs
+    import oracle
+
+    B.lib()
+    rng = np.random.default_rng(7)
+    shapes = []
+    for _ in range(args.batch_count):
+        m = int(2 ** rng.uniform(12, 20))
+        shapes.append((m, int(rng.choice([32, 64, 128])), int(rng.choice([4, 8, 16])),
+                       float(10 ** rng.uniform(2, 10))))
+    log("generating %d instances" % len(shapes))
+    X0 = [make_input(torch, m, n, kappa, seed=100 + i) for i, (m, n, s, kappa) in
+          enumerate(shapes)]
+    X = [t.clone() for t in X0]
+    R = [torch.zeros(n * n, dtype=torch.float64, device="cuda") for (m, n, s, k) in shapes]
+    ms_ = [m for (m, n, s, k) in shapes]
+    ns_ = [n for (m, n, s, k) in shapes]
+    ss_ = [s for (m, n, s, k) in shapes]
+    small = sorted(range(len(shapes)), key=lambda i: shapes[i][0] * shapes[i][1])[:8]
+    oracle.set_threads(args.cpu_threads)
+
+    def restore(idx):
+        for i in idx:
+            X[i].copy_(X0[i])
+            R[i].zero_()
+        torch.cuda.synchronize()
+
+    def measure(name, idx, call):
+        restore(idx)
+        log("batched %s: tuning on %d instances" % (name, len(idx)))
+        info, cfg = call(True)          # tune (captures and times candidates)
+        log("batched %s: tuned %s" % (name, cfg))
+        times = []
+        for _ in range(args.batch_reps):
+            restore(idx)
+            t0 = time.perf_counter()
+            info, cfg = call(False)     # graph replay
+            times.append(time.perf_counter() - t0)
+        t = sorted(times)[len(times) // 2]
+        byt = sum(b_alg(shapes[i][0], shapes[i][1], shapes[i][2]) for i in idx)
+        loos, brk = [], 0
+        for k, i in enumerate(idx):
+            m, n, s, kappa = shapes[i]
+            if info[k][0]:
+                brk += 1
+                continue
+            lo, re_, xn = B.accuracy_device(m, n, X0[i].data_ptr(), m, X[i].data_ptr(), m,
+                                            R[i].data_ptr(), n)
+            loos.append(lo)
+        d = {"config": "batched", "case": name, "instances": len(idx), "ms": round(t * 1e3, 2),
+             "GBs_alg": round(byt / t / 1e9, 1), "streams": cfg[0], "wg_share": cfg[1],
+             "graphs": cfg[2],
+             "breakdowns": brk,
+             "loo_median": float(np.median(loos)) if loos else None,
+             "loo_max": float(max(loos)) if loos else None}
+        return d, info
+
+    for variant in ("bcgsi2P1s", "bcgsi2"):
+        idx = list(range(len(shapes)))
+        d, info = measure(variant, idx, lambda tune: B.bcgs_batched_device(
+            variant, ms_, ns_, ss_, [t.data_ptr() for t in X], ms_, [r.data_ptr() for r in R],
+            ns_, tune=tune))
+        ol = []
+        for i in small:
+            m, n, s, kappa = shapes[i]
+            Qo, Ro, info_o = oracle.bcgs(variant, X0[i].cpu().numpy().T, s)
+            if info_o == (0, 0) and info[i] == (0, 0):
+                lo, _, _ = B.accuracy_device(m, n, X0[i].data_ptr(), m, X[i].data_ptr(), m,
+                                             R[i].data_ptr(), n)
+                ol.append((lo, inputs.loo(Qo)))
+        d["oracle_sample"] = [{"gpu_loo": a, "cpu_loo": b} for a, b in ol]
+        print(json.dumps(finite(d)), flush=True)
+
+    # Mixed form on the s = 16 instances (its supported shapes).
+    idx = [i for i in range(len(shapes)) if shapes[i][2] == 16]
+    X32 = {i: X0[i].float() for i in idx}
+    for prec in ("fp64", "bf16", "bf16x3"):
+        d, _ = measure("bcgsi2P1s-" + prec, idx, lambda tune: B.bcgs_batched_device(
+            "bcgsi2P1s", [ms_[i] for i in idx], [ns_[i] for i in idx], [16] * len(idx),
+            [X[i].data_ptr() for i in idx], [ms_[i] for i in idx],
+            [R[i].data_ptr() for i in idx], [ns_[i] for i in idx], tune=tune,
+            precision=prec, x32_ptrs=[X32[i].data_ptr() for i in idx],
+            ldx32s=[ms_[i] for i in idx]))
+        print(json.dumps(finite(d)), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -244,6 +342,11 @@ def main():
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-mixed", action="store_true",
                     help="skip the mixed-precision leg (FP32-stored input)")
+    ap.add_argument("--batched", action="store_true",
+                    help="run the batched form on an uneven batch instead of the headline "
+                         "line (one JSON object per case)")
+    ap.add_argument("--batch-count", type=int, default=512)
+    ap.add_argument("--batch-reps", type=int, default=3)
     ap.add_argument("--suite", action="store_true",
                     help="run the per-variant / per-kappa / glued measurement cases "
                          "instead of the headline line (one JSON object per case)")
@@ -260,6 +363,9 @@ def main():
 
     if args.suite:
         suite(args)
+        return
+    if args.batched:
+        batched(args)
         return
 
     import torch

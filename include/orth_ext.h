@@ -1,7 +1,7 @@
 /*
  * Extended interface of liborth_hip.so: device-pointer and stream entry
- * points, status reporting, the accuracy harness and the mixed-precision
- * form.  The five reference-compatible symbols are in
+ * points, status reporting, the accuracy harness, the mixed-precision
+ * form and the batched form.  The five reference-compatible symbols are in
  * orth.h; nothing here has a counterpart in the reference's include/orth.h
  * unless its comment cites one.
  *
@@ -72,6 +72,40 @@ int orth_bcgs_mixed_device(int variant, int precision, int64_t m, int64_t n,
                            int64_t s, const float *dX32, int64_t ldX32,
                            double *dQ, int64_t ldQ, double *dR, int64_t ldR,
                            void *stream);
+
+/* Batched form (no counterpart in the reference): `count` independent
+ * orthogonalizations of uneven shapes, each as orth_bcgs_device (instance i:
+ * m[i] x n[i] with block width s[i], device dX[i] (ld ldX[i]) becomes Q,
+ * device dR[i] (ld ldR[i]) receives R).  Instances run side by side on
+ * several internal streams, each with private scratch; the batch is
+ * captured into hipGraphs and replayed while the same batch (shapes and
+ * pointers) is passed again.  tune != 0 first times the stream count and
+ * workgroup share on this batch and keeps the best for batches with the
+ * same shapes; it also decides between replaying captured graphs and
+ * launching directly on the streams.  Batches larger than a graph-size
+ * limit are captured as several graphs launched back to back.  info (host,
+ * 2 * count ints, may be NULL) receives each instance's breakdown (block,
+ * pivot); config (host, 2 ints, may be NULL) the number of streams used
+ * (negative when launched directly rather than from graphs) and the
+ * workgroup share in thousandths.
+ * Synchronous (waits for `stream` first).  Returns 0, or 2 for an invalid
+ * instance or a HIP error. */
+int orth_bcgs_batched_device(int variant, int count, const int64_t *m,
+                             const int64_t *n, const int64_t *s,
+                             double *const *dX, const int64_t *ldX,
+                             double *const *dR, const int64_t *ldR, int *info,
+                             int tune, int *config, void *stream);
+
+/* The batched form of orth_bcgs_mixed_device: FP32 inputs dX32[i], FP64
+ * outputs dQ[i] and dR[i]; every instance must satisfy that function's
+ * limits (ORTH_P1S, s = 16, n <= 256). */
+int orth_bcgs_batched_mixed_device(int variant, int precision, int count,
+                                   const int64_t *m, const int64_t *n,
+                                   const int64_t *s, const float *const *dX32,
+                                   const int64_t *ldX32, double *const *dQ,
+                                   const int64_t *ldQ, double *const *dR,
+                                   const int64_t *ldR, int *info, int tune,
+                                   int *config, void *stream);
 
 /* Accuracy harness of the reference (SRC/util.c:36-75) on device data:
  * out[0] = ||Q^T Q - I||_F, out[1] = ||Q R - X||_F, out[2] = ||X||_F.

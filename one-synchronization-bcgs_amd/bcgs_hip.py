@@ -80,11 +80,21 @@ def lib():
     L.orth_kernel_times.restype = ctypes.c_int
     L.orth_kernel_times.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                                     ctypes.POINTER(ctypes.c_int), _dp, _dp]
-    if hasattr(L, "orth_bcgs_mixed_device"):
-        L.orth_bcgs_mixed_device.restype = ctypes.c_int
-        L.orth_bcgs_mixed_device.argtypes = [ctypes.c_int, ctypes.c_int, _i64,
-                                             _i64, _i64, _vp, _i64, _vp, _i64,
-                                             _vp, _i64, _vp]
+    L.orth_bcgs_mixed_device.restype = ctypes.c_int
+    L.orth_bcgs_mixed_device.argtypes = [ctypes.c_int, ctypes.c_int, _i64, _i64, _i64, _vp,
+                                         _i64, _vp, _i64, _vp, _i64, _vp]
+    _ip = ctypes.POINTER(ctypes.c_int)
+    _i64p = ctypes.POINTER(ctypes.c_int64)
+    _vpp = ctypes.POINTER(ctypes.c_void_p)
+    L.orth_bcgs_batched_device.restype = ctypes.c_int
+    L.orth_bcgs_batched_device.argtypes = [ctypes.c_int, ctypes.c_int, _i64p, _i64p, _i64p,
+                                           _vpp, _i64p, _vpp, _i64p, _ip, ctypes.c_int, _ip,
+                                           _vp]
+    L.orth_bcgs_batched_mixed_device.restype = ctypes.c_int
+    L.orth_bcgs_batched_mixed_device.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                 _i64p, _i64p, _i64p, _vpp, _i64p, _vpp,
+                                                 _i64p, _vpp, _i64p, _ip, ctypes.c_int, _ip,
+                                                 _vp]
     _lib = L
     return L
 
@@ -144,6 +154,38 @@ def bcgs_mixed_device(variant, precision, m, n, s, x32_ptr, ldx, q_ptr, ldq,
                                       r_ptr, ldr, stream or None)
     if rc != 0:
         raise RuntimeError("orth_bcgs_mixed_device returned %d" % rc)
+
+
+def _arr(ctype, vals):
+    return (ctype * len(vals))(*vals)
+
+
+def bcgs_batched_device(variant, ms, ns, ss, x_ptrs, ldxs, r_ptrs, ldrs, tune=False,
+                        stream=0, precision=None, x32_ptrs=None, ldx32s=None):
+    """Run a batch of independent orthogonalizations on device pointers
+    (synchronous).  With precision set, x32_ptrs are the FP32 inputs and
+    x_ptrs the FP64 outputs Q.  Returns (info, config): info[i] = (block,
+    pivot) of instance i's first breakdown, config = (streams, workgroup
+    share, graphs used) of the run."""
+    cnt = len(ms)
+    info = (ctypes.c_int * (2 * cnt))()
+    cfg = (ctypes.c_int * 2)()
+    i64 = ctypes.c_int64
+    vp = ctypes.c_void_p
+    common = (_arr(i64, ms), _arr(i64, ns), _arr(i64, ss))
+    if precision is None:
+        rc = lib().orth_bcgs_batched_device(VARIANTS[variant], cnt, *common, _arr(vp, x_ptrs),
+                                            _arr(i64, ldxs), _arr(vp, r_ptrs), _arr(i64, ldrs),
+                                            info, 1 if tune else 0, cfg, stream or None)
+    else:
+        rc = lib().orth_bcgs_batched_mixed_device(
+            VARIANTS[variant], PRECISIONS[precision], cnt, *common, _arr(vp, x32_ptrs),
+            _arr(i64, ldx32s), _arr(vp, x_ptrs), _arr(i64, ldxs), _arr(vp, r_ptrs),
+            _arr(i64, ldrs), info, 1 if tune else 0, cfg, stream or None)
+    if rc != 0:
+        raise RuntimeError("batched call returned %d" % rc)
+    return ([(info[2 * i], info[2 * i + 1]) for i in range(cnt)],
+            (abs(cfg[0]), cfg[1] / 1000.0, cfg[0] > 0))
 
 
 def accuracy_device(m, n, x_ptr, ldx, q_ptr, ldq, r_ptr, ldr, stream=0):

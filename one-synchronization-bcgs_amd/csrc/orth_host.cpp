@@ -187,7 +187,7 @@ class Run {
     // P <- [X(:, a0:a0+na)]^T [X(:, b0:b0+nb)]; returns the buffer (ld na).
     double *proj(int64_t a0, int64_t na, int64_t b0, int64_t nb)
     {
-        ProjPlan pl = plan_proj(m_, na, nb, kProjTargetWG);
+        ProjPlan pl = plan_proj(m_, na, nb, std::max(1, (int)(kProjTargetWG * wg_scale_)));
         const size_t need = (size_t)pl.nwg * na * nb;
         if (!ensure(c_.part, c_.part_n, need)) return nullptr;
         if (pl.lds_bytes > 64 * 1024) return nullptr;
@@ -474,6 +474,10 @@ class Run {
     }
     bool lp_first_pass() const { return mx_ != 0 && mx_ != kMxFP64; }
 
+    // Fraction of the device's workgroups one pass uses (the batched form
+    // runs several calls side by side).
+    void set_wg_scale(double f) { wg_scale_ = std::min(1.0, std::max(f, 1e-3)); }
+
     // One fused pass: the pending update, then P <- A^T B (na == 0: none).
     double *pass(int64_t a0, int64_t na, int64_t b0, int64_t nb)
     {
@@ -541,7 +545,8 @@ class Run {
         // The mixed pass with reduced-precision products runs one 512-thread
         // workgroup per CU (its operand conversions need two waves' worth of
         // registers); with FP64 products it runs two, like the FP64 pass.
-        int nwg = std::max(1, c_.ncu) * (ts16 > 0 || mx_ == kMxFP64 ? 2 : 1);
+        int nwg = std::max(
+            1, (int)(std::max(1, c_.ncu) * (ts16 > 0 || mx_ == kMxFP64 ? 2 : 1) * wg_scale_));
         int64_t rows = (m_ + nwg - 1) / nwg;
         rows = (rows + unit - 1) / unit * unit;
         pa.rows_per = rows;
@@ -782,6 +787,7 @@ class Run {
     int64_t ldx32_ = 0;
     int64_t frontier_ = 0;      // columns [0, frontier_) have been written to X
     int64_t lp_from_ = -1;      // next pass: B columns from here on are reduced precision
+    double wg_scale_ = 1.0;
 
     // Debug aid (ORTH_DEBUG_DUMP set): print the reduced projection and the
     // factors after each small step.
@@ -916,6 +922,377 @@ int dropin(int variant, int m, int n, int s, double *X, int ldX, double *R, int 
     return 0;
 }
 
+// ---------------------------------------------------------------------------
+// Batched form: many independent calls side by side.
+//
+// Instances are assigned to up to kMaxSlots streams (longest first, by
+// algorithmic bytes); each stream owns a private scratch set (a DevCtx) and
+// runs its instances one after the other.  The whole fork / per-stream
+// sequences / join is captured once into a hipGraph and replayed while the
+// same batch (shapes and pointers) comes back.  The stream count and the
+// share of the device's workgroups each pass uses are tunable; the tuner
+// times the candidates on the batch itself and keeps the best per shape
+// signature.
+// ---------------------------------------------------------------------------
+constexpr int kMaxSlots = 8;
+
+struct BatchCfg {
+    int streams = 4;
+    double wg_scale = 0.5;
+    bool graph = true;  // capture into hipGraphs (else launch directly on the streams)
+};
+
+// Graph nodes per captured graph at most: batches beyond it are captured as
+// several graphs launched back to back.
+constexpr int64_t kMaxGraphNodes = 12000;
+
+struct BatchCtx {
+    bool init = false;
+    DevCtx slot[kMaxSlots];
+    hipStream_t st[kMaxSlots] = {};
+    hipStream_t main = nullptr;
+    hipEvent_t fork = nullptr;
+    hipEvent_t join[kMaxSlots] = {};
+    int *info_d = nullptr;
+    int *info_h = nullptr;
+    size_t info_n = 0;
+    std::vector<hipGraphExec_t> execs;  // the captured batch, in launch order
+    std::vector<int64_t> key;           // batch the graphs were captured for
+    BatchCfg key_cfg;
+    std::vector<std::pair<std::vector<int64_t>, BatchCfg>> tuned;  // shape signature -> config
+};
+
+BatchCtx g_batch[kMaxDevices];
+
+struct BatchArgs {
+    int variant = 0;
+    int mx = 0;
+    int count = 0;
+    const int64_t *m = nullptr, *n = nullptr, *s = nullptr;
+    double *const *X = nullptr;
+    const int64_t *ldX = nullptr;
+    double *const *R = nullptr;
+    const int64_t *ldR = nullptr;
+    const float *const *X32 = nullptr;
+    const int64_t *ldX32 = nullptr;
+};
+
+bool batch_init(BatchCtx &b)
+{
+    if (b.init) return true;
+    for (int k = 0; k < kMaxSlots; k++) {
+        if (!ctx_init(b.slot[k])) return false;
+        if (!ok(hipStreamCreateWithFlags(&b.st[k], hipStreamNonBlocking))) return false;
+        if (!ok(hipEventCreateWithFlags(&b.join[k], hipEventDisableTiming))) return false;
+    }
+    if (!ok(hipStreamCreateWithFlags(&b.main, hipStreamNonBlocking))) return false;
+    if (!ok(hipEventCreateWithFlags(&b.fork, hipEventDisableTiming))) return false;
+    b.init = true;
+    return true;
+}
+
+double inst_bytes(int64_t m, int64_t n, int64_t s)
+{
+    const int64_t N = 1 + (n - s + s - 1) / s;
+    return 8.0 * m * (2.0 * n + (double)s * N * (N - 1) / 2);
+}
+
+std::vector<int64_t> shape_sig(const BatchArgs &a)
+{
+    std::vector<int64_t> k = {a.variant, a.mx, a.count};
+    for (int i = 0; i < a.count; i++) {
+        k.push_back(a.m[i]);
+        k.push_back(a.n[i]);
+        k.push_back(a.s[i]);
+    }
+    return k;
+}
+
+// Graph nodes of one instance: per synchronization a pass, a reduction and
+// a small step; the fills, the R copies and the status copy.
+int64_t inst_nodes(int variant, int64_t n, int64_t s)
+{
+    const int64_t N = 1 + (n - s + s - 1) / s;
+    int64_t syncs = 3 + N;
+    if (variant == ORTH_PIPI) syncs = 3 + 2 * (N - 1);
+    if (variant == ORTH_I2) syncs = 3 + 8 * (N - 1);
+    if (variant == ORTH_P2S) syncs = 7 + (N - 1) + 3 * std::max<int64_t>(0, N - 2);
+    return 3 * syncs + N + 8;
+}
+
+std::vector<int64_t> full_key(const BatchArgs &a, const BatchCfg &cfg)
+{
+    std::vector<int64_t> k = shape_sig(a);
+    k.push_back(cfg.graph ? 1 : 0);
+    k.push_back(cfg.streams);
+    k.push_back((int64_t)(cfg.wg_scale * 1e6));
+    for (int i = 0; i < a.count; i++) {
+        k.push_back((int64_t)(intptr_t)a.X[i]);
+        k.push_back(a.ldX[i]);
+        k.push_back((int64_t)(intptr_t)a.R[i]);
+        k.push_back(a.ldR[i]);
+        if (a.mx) {
+            k.push_back((int64_t)(intptr_t)a.X32[i]);
+            k.push_back(a.ldX32[i]);
+        }
+    }
+    return k;
+}
+
+// Longest-first assignment of instances (indices idx) to streams.
+std::vector<std::vector<int>> assign(const BatchArgs &a, const std::vector<int> &idx, int streams)
+{
+    std::vector<int> order = idx;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+        return inst_bytes(a.m[x], a.n[x], a.s[x]) > inst_bytes(a.m[y], a.n[y], a.s[y]);
+    });
+    std::vector<std::vector<int>> lists(streams);
+    std::vector<double> load(streams, 0.0);
+    for (int i : order) {
+        const int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        lists[k].push_back(i);
+        load[k] += inst_bytes(a.m[i], a.n[i], a.s[i]);
+    }
+    return lists;
+}
+
+bool ensure_info(BatchCtx &b, int count)
+{
+    if (b.info_n >= (size_t)count * 2) return true;
+    if (b.info_d) (void)hipFree(b.info_d);
+    if (b.info_h) (void)hipHostFree(b.info_h);
+    b.info_d = nullptr;
+    b.info_h = nullptr;
+    b.info_n = 0;
+    if (!ok(hipMalloc(&b.info_d, (size_t)count * 2 * sizeof(int)))) return false;
+    if (!ok(hipHostMalloc(&b.info_h, (size_t)count * 2 * sizeof(int), hipHostMallocDefault)))
+        return false;
+    b.info_n = (size_t)count * 2;
+    return true;
+}
+
+// Allocate every slot's scratch for its largest instance (no allocation may
+// happen while the sequence is captured).
+bool batch_reserve(BatchCtx &b, const BatchArgs &a, const std::vector<std::vector<int>> &lists)
+{
+    for (size_t k = 0; k < lists.size(); k++) {
+        int64_t nmax = 0;
+        int im = -1;
+        for (int i : lists[k])
+            if (a.n[i] > nmax) {
+                nmax = a.n[i];
+                im = i;
+            }
+        if (im < 0) continue;
+        Run r(b.slot[k], a.m[im], a.n[im], a.s[im], a.X[im], a.ldX[im], b.st[k]);
+        if (!r.prepare()) return false;
+    }
+    if (!ensure_info(b, a.count)) return false;
+    return ok(hipDeviceSynchronize());
+}
+
+// Enqueue instances on b.main (fork, per-stream sequences, join); `last`
+// adds the copy of every instance's status to the host.
+bool batch_enqueue(BatchCtx &b, const BatchArgs &a, const BatchCfg &cfg,
+                   const std::vector<std::vector<int>> &lists, bool last)
+{
+    if (!ok(hipEventRecord(b.fork, b.main))) return false;
+    for (int k = 0; k < (int)lists.size(); k++) {
+        hipStream_t st = b.st[k];
+        if (!ok(hipStreamWaitEvent(st, b.fork, 0))) return false;
+        for (int i : lists[k]) {
+            Run r(b.slot[k], a.m[i], a.n[i], a.s[i], a.X[i], a.ldX[i], st);
+            r.set_wg_scale(cfg.wg_scale);
+            if (a.mx) r.set_mixed(a.mx, a.X32[i], a.ldX32[i]);
+            if (!r.prepare() || !r.run(a.variant)) return false;
+            if (!r.copy_R_out(a.R[i], a.ldR[i], hipMemcpyDeviceToDevice)) return false;
+            if (!ok(hipMemcpyAsync(b.info_d + 2 * i, b.slot[k].status_d, 2 * sizeof(int),
+                                   hipMemcpyDeviceToDevice, st)))
+                return false;
+        }
+        if (!ok(hipEventRecord(b.join[k], st))) return false;
+        if (!ok(hipStreamWaitEvent(b.main, b.join[k], 0))) return false;
+    }
+    if (!last) return true;
+    return ok(hipMemcpyAsync(b.info_h, b.info_d, (size_t)a.count * 2 * sizeof(int),
+                             hipMemcpyDeviceToHost, b.main));
+}
+
+// Instance chunks of at most kMaxGraphNodes graph nodes each.
+std::vector<std::vector<int>> graph_chunks(const BatchArgs &a)
+{
+    std::vector<std::vector<int>> chunks(1);
+    int64_t nodes = 0;
+    for (int i = 0; i < a.count; i++) {
+        const int64_t c = inst_nodes(a.variant, a.n[i], a.s[i]);
+        if (nodes + c > kMaxGraphNodes && !chunks.back().empty()) {
+            chunks.push_back({});
+            nodes = 0;
+        }
+        chunks.back().push_back(i);
+        nodes += c;
+    }
+    return chunks;
+}
+
+void drop_graphs(BatchCtx &b)
+{
+    for (hipGraphExec_t e : b.execs) (void)hipGraphExecDestroy(e);
+    b.execs.clear();
+    b.key.clear();
+}
+
+// Run the batch with cfg.  Graph mode replays the captured graphs when the
+// batch is the one they were captured for, else captures them (falling
+// back to direct launches if capture fails); direct mode launches on the
+// streams.
+bool batch_run(BatchCtx &b, const BatchArgs &a, const BatchCfg &cfg)
+{
+    std::vector<int> all(a.count);
+    for (int i = 0; i < a.count; i++) all[i] = i;
+    if (!cfg.graph) {
+        // Direct launches may grow the slots' scratch: captured graphs that
+        // refer to the old buffers are dropped first.
+        drop_graphs(b);
+        if (!ensure_info(b, a.count)) return false;
+        if (!batch_enqueue(b, a, cfg, assign(a, all, cfg.streams), true)) return false;
+        return ok(hipStreamSynchronize(b.main));
+    }
+    const std::vector<int64_t> key = full_key(a, cfg);
+    if (b.execs.empty() || key != b.key) {
+        drop_graphs(b);
+        const auto chunks = graph_chunks(a);
+        bool good = true;
+        for (size_t ci = 0; ci < chunks.size() && good; ci++) {
+            const auto lists = assign(a, chunks[ci], cfg.streams);
+            if (!batch_reserve(b, a, lists)) return false;
+            hipGraph_t g = nullptr;
+            good = ok(hipStreamBeginCapture(b.main, hipStreamCaptureModeThreadLocal));
+            if (good) {
+                const bool enq = batch_enqueue(b, a, cfg, lists, ci + 1 == chunks.size());
+                good = ok(hipStreamEndCapture(b.main, &g)) && enq && g;
+            }
+            hipGraphExec_t e = nullptr;
+            if (good && ok(hipGraphInstantiate(&e, g, nullptr, nullptr, 0)))
+                b.execs.push_back(e);
+            else
+                good = false;
+            if (g) (void)hipGraphDestroy(g);
+        }
+        if (!good) {
+            drop_graphs(b);
+            (void)hipGetLastError();
+            tl_hip_error = 0;
+            BatchCfg direct = cfg;
+            direct.graph = false;
+            return batch_run(b, a, direct);
+        }
+        b.key = key;
+    }
+    for (hipGraphExec_t e : b.execs)
+        if (!ok(hipGraphLaunch(e, b.main))) return false;
+    return ok(hipStreamSynchronize(b.main));
+}
+
+BatchCfg batch_cfg(const BatchCtx &b, const BatchArgs &a)
+{
+    const std::vector<int64_t> sig = shape_sig(a);
+    for (const auto &t : b.tuned)
+        if (t.first == sig) return t.second;
+    BatchCfg c;
+    c.streams = std::min(kMaxSlots, std::max(1, a.count));
+    c.streams = std::min(c.streams, 4);
+    return c;
+}
+
+bool batch_args_ok(const BatchArgs &a)
+{
+    if (a.count < 1 || !a.m || !a.n || !a.s || !a.X || !a.ldX || !a.R || !a.ldR) return false;
+    if (a.mx && (!a.X32 || !a.ldX32)) return false;
+    for (int i = 0; i < a.count; i++) {
+        if (!a.X[i] || !a.R[i] || !shape_ok(a.m[i], a.n[i], a.s[i], a.ldX[i], a.ldR[i])) return false;
+        if (a.mx && (!a.X32[i] || a.ldX32[i] < a.m[i] || a.variant != ORTH_P1S || a.s[i] != 16 ||
+                     a.n[i] > 256))
+            return false;
+    }
+    return a.variant >= 0 && a.variant <= 4;
+}
+
+int batched_entry(const BatchArgs &a, int *info, void *stream, int tune, int *cfg_out)
+{
+    if (!batch_args_ok(a)) return 2;
+    tl_hip_error = 0;
+    const int dev = current_device();
+    DevCtx &c = g_ctx[dev];
+    std::lock_guard<std::mutex> lk(c.mu);
+    if (!ctx_init(c)) return 2;
+    BatchCtx &b = g_batch[dev];
+    if (!batch_init(b)) return 2;
+    // Order after the caller's stream and the library's earlier calls.
+    if (!ok(hipStreamSynchronize((hipStream_t)stream))) return 2;
+    if (c.done_valid && !ok(hipEventSynchronize(c.done))) return 2;
+    BatchCfg cfg = batch_cfg(b, a);
+    if (tune) {
+        hipEvent_t e0, e1;
+        if (!ok(hipEventCreate(&e0)) || !ok(hipEventCreate(&e1))) return 2;
+        float best = 1e30f;
+        BatchCfg bestc = cfg;
+        // Candidates: (streams, workgroup share) from one stream owning the
+        // device to eight streams sharing it.
+        const std::pair<int, double> cands[] = {{2, 0.5}, {4, 0.5}, {8, 0.25}};
+        for (const bool graph : {true, false}) {
+            for (const auto &cd : cands) {
+                if (cd.first > a.count) continue;
+                BatchCfg cand;
+                cand.streams = cd.first;
+                cand.wg_scale = cd.second;
+                cand.graph = graph;
+                if (graph && !batch_run(b, a, cand)) return 2;  // capture
+                if (!ok(hipEventRecord(e0, b.main))) return 2;
+                if (!batch_run(b, a, cand)) return 2;
+                if (!ok(hipEventRecord(e1, b.main)) || !ok(hipEventSynchronize(e1))) return 2;
+                float ms = 0.f;
+                (void)hipEventElapsedTime(&ms, e0, e1);
+                if (ms < best) {
+                    best = ms;
+                    bestc = cand;
+                }
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        const std::vector<int64_t> sig = shape_sig(a);
+        bool found = false;
+        for (auto &t : b.tuned)
+            if (t.first == sig) {
+                t.second = bestc;
+                found = true;
+            }
+        if (!found) b.tuned.push_back({sig, bestc});
+        cfg = bestc;
+    }
+    if (!batch_run(b, a, cfg)) return 2;
+    int worst = 0;
+    for (int i = 0; i < a.count; i++) {
+        if (info) {
+            info[2 * i] = b.info_h[2 * i];
+            info[2 * i + 1] = b.info_h[2 * i + 1];
+        }
+        if (b.info_h[2 * i] && !worst) worst = i + 1;
+    }
+    // orth_last_status reports the first instance that broke down.
+    c.status_h[0] = worst ? b.info_h[2 * (worst - 1)] : 0;
+    c.status_h[1] = worst ? b.info_h[2 * (worst - 1) + 1] : 0;
+    c.last_passes = 0;
+    c.last_syncs = 0;
+    c.done_valid = false;
+    if (cfg_out) {
+        cfg_out[0] = cfg.streams * (cfg.graph ? 1 : -1);
+        cfg_out[1] = (int)(cfg.wg_scale * 1000 + 0.5);
+    }
+    return 0;
+}
+
 }  // namespace
 }  // namespace orth
 
@@ -985,6 +1362,52 @@ int orth_bcgs_mixed_device(int variant, int precision, int64_t m, int64_t n, int
     if (!ctx_init(c)) return 2;
     return run_device(c, variant, m, n, s, dQ, ldQ, dR, ldR, hipMemcpyDeviceToDevice,
                       (hipStream_t)stream, mx, dX32, ldX32);
+}
+
+int orth_bcgs_batched_device(int variant, int count, const int64_t *m, const int64_t *n,
+                             const int64_t *s, double *const *dX, const int64_t *ldX,
+                             double *const *dR, const int64_t *ldR, int *info, int tune,
+                             int *config, void *stream)
+{
+    BatchArgs a;
+    a.variant = variant;
+    a.count = count;
+    a.m = m;
+    a.n = n;
+    a.s = s;
+    a.X = dX;
+    a.ldX = ldX;
+    a.R = dR;
+    a.ldR = ldR;
+    return batched_entry(a, info, stream, tune, config);
+}
+
+int orth_bcgs_batched_mixed_device(int variant, int precision, int count, const int64_t *m,
+                                   const int64_t *n, const int64_t *s, const float *const *dX32,
+                                   const int64_t *ldX32, double *const *dQ, const int64_t *ldQ,
+                                   double *const *dR, const int64_t *ldR, int *info, int tune,
+                                   int *config, void *stream)
+{
+    BatchArgs a;
+    a.variant = variant;
+    a.count = count;
+    a.m = m;
+    a.n = n;
+    a.s = s;
+    a.X = dQ;
+    a.ldX = ldQ;
+    a.R = dR;
+    a.ldR = ldR;
+    a.X32 = dX32;
+    a.ldX32 = ldX32;
+    switch (precision) {
+    case ORTH_PREC_FP64: a.mx = kMxFP64; break;
+    case ORTH_PREC_BF16: a.mx = kMxBF16; break;
+    case ORTH_PREC_FP16: a.mx = kMxFP16; break;
+    case ORTH_PREC_BF16X3: a.mx = kMxBF16X3; break;
+    default: return 2;
+    }
+    return batched_entry(a, info, stream, tune, config);
 }
 
 int orth_bcgs_device(int variant, int64_t m, int64_t n, int64_t s, double *dX, int64_t ldX,

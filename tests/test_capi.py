@@ -92,3 +92,19 @@ def test_mixed_rejects_unsupported_without_device(args):
     rc = L.orth_bcgs_mixed_device(0, 7, m, 32, 16, ctypes.c_void_p(16), m,
                                   ctypes.c_void_p(16), m, ctypes.c_void_p(16), 32, None)
     assert rc == 2
+
+
+def test_batched_rejects_invalid_without_device():
+    i64 = ctypes.c_int64
+    vp = ctypes.c_void_p
+    L = B.lib()
+    arr = lambda t, v: (t * len(v))(*v)
+    # s > n for the second instance
+    rc = L.orth_bcgs_batched_device(0, 2, arr(i64, [100, 100]), arr(i64, [16, 8]),
+                                    arr(i64, [4, 16]), arr(vp, [16, 16]), arr(i64, [100, 100]),
+                                    arr(vp, [16, 16]), arr(i64, [16, 16]), None, 0, None, None)
+    assert rc == 2
+    rc = L.orth_bcgs_batched_device(0, 0, arr(i64, [1]), arr(i64, [1]), arr(i64, [1]),
+                                    arr(vp, [16]), arr(i64, [1]), arr(vp, [16]), arr(i64, [1]),
+                                    None, 0, None, None)
+    assert rc == 2
