@@ -1071,19 +1071,15 @@ bool ensure_info(BatchCtx &b, int count)
     return true;
 }
 
-// Allocate every slot's scratch for its largest instance (no allocation may
-// happen while the sequence is captured).
-bool batch_reserve(BatchCtx &b, const BatchArgs &a, const std::vector<std::vector<int>> &lists)
+// Size every slot's scratch for the widest instance of the batch before
+// anything is captured: a captured graph keeps the buffer addresses, so no
+// slot buffer may be reallocated while graphs of this batch exist.
+bool batch_reserve(BatchCtx &b, const BatchArgs &a, int streams)
 {
-    for (size_t k = 0; k < lists.size(); k++) {
-        int64_t nmax = 0;
-        int im = -1;
-        for (int i : lists[k])
-            if (a.n[i] > nmax) {
-                nmax = a.n[i];
-                im = i;
-            }
-        if (im < 0) continue;
+    int im = 0;
+    for (int i = 1; i < a.count; i++)
+        if (a.n[i] > a.n[im]) im = i;
+    for (int k = 0; k < streams; k++) {
         Run r(b.slot[k], a.m[im], a.n[im], a.s[im], a.X[im], a.ldX[im], b.st[k]);
         if (!r.prepare()) return false;
     }
@@ -1154,18 +1150,18 @@ bool batch_run(BatchCtx &b, const BatchArgs &a, const BatchCfg &cfg)
         // Direct launches may grow the slots' scratch: captured graphs that
         // refer to the old buffers are dropped first.
         drop_graphs(b);
-        if (!ensure_info(b, a.count)) return false;
+        if (!batch_reserve(b, a, cfg.streams)) return false;
         if (!batch_enqueue(b, a, cfg, assign(a, all, cfg.streams), true)) return false;
         return ok(hipStreamSynchronize(b.main));
     }
     const std::vector<int64_t> key = full_key(a, cfg);
     if (b.execs.empty() || key != b.key) {
         drop_graphs(b);
+        if (!batch_reserve(b, a, cfg.streams)) return false;
         const auto chunks = graph_chunks(a);
         bool good = true;
         for (size_t ci = 0; ci < chunks.size() && good; ci++) {
             const auto lists = assign(a, chunks[ci], cfg.streams);
-            if (!batch_reserve(b, a, lists)) return false;
             hipGraph_t g = nullptr;
             good = ok(hipStreamBeginCapture(b.main, hipStreamCaptureModeThreadLocal));
             if (good) {
