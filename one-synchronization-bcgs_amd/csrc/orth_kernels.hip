@@ -1014,7 +1014,9 @@ __device__ __forceinline__ dbl4 zsteps(const double *sm, int ao, int bo, int LD,
     return z;
 }
 
-template <int MT, int NB, int F>
+// EARLY: passes without a projection issue the next tile's loads as soon as
+// the current tile is in LDS (nothing else would hide their latency).
+template <int MT, int NB, int F, bool EARLY>
 __global__ __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_pass16b(PassArgs a)
 {
     constexpr int N1 = 32, TS = 32 * F, LD = TS + 1;
@@ -1087,6 +1089,9 @@ __global__ __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(4, 4))) void
         for (int i = 0; i < kBPF; i++)
             if (i < nld) sm[pl + i * PSTEP * LD] = pf[i];
         __syncthreads();
+        if constexpr (EARLY) {
+            if (r0 + TS < re) load(r0 + TS);
+        }
 
         if (a.ngrp > 0) {
             const int nphase = a.seq ? a.ngrp : 1;
@@ -1232,7 +1237,9 @@ __global__ __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(4, 4))) void
         }
         // Next tile's loads: issued after the triangular solve, so the
         // prefetch registers are live only across the projection.
-        if (r0 + TS < re) load(r0 + TS);
+        if constexpr (!EARLY) {
+            if (r0 + TS < re) load(r0 + TS);
+        }
 
         if (doproj) {
 #pragma unroll
@@ -1267,19 +1274,31 @@ __global__ __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(4, 4))) void
     }
 }
 
-template <int MT, int NB, int F>
+template <int MT, int NB, int F, bool EARLY = false>
 hipError_t launch_pass16b_t(const PassArgs &a, size_t lds, int nwg, hipStream_t st)
 {
     static bool attr_set = false;
     if (!attr_set) {
-        hipError_t e = hipFuncSetAttribute((const void *)k_pass16b<MT, NB, F>,
+        hipError_t e = hipFuncSetAttribute((const void *)k_pass16b<MT, NB, F, EARLY>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize,
                                            (int)(160 * 1024));
         if (e != hipSuccess) return e;
         attr_set = true;
     }
-    k_pass16b<MT, NB, F><<<dim3(nwg), dim3(kB), lds, st>>>(a);
+    k_pass16b<MT, NB, F, EARLY><<<dim3(nwg), dim3(kB), lds, st>>>(a);
     return hipGetLastError();
+}
+
+// Passes without a projection.
+hipError_t launch_pass16b_noproj(const PassArgs &a, int ts, size_t lds, int nwg, hipStream_t st)
+{
+    switch (ts) {
+    case 32: return launch_pass16b_t<1, 1, 1, true>(a, lds, nwg, st);
+    case 64: return launch_pass16b_t<1, 1, 2, true>(a, lds, nwg, st);
+    case 128: return launch_pass16b_t<1, 1, 4, true>(a, lds, nwg, st);
+    case 256: return launch_pass16b_t<1, 1, 8, true>(a, lds, nwg, st);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 // Only the (MT, F) pairs the plan can produce are instantiated: a tile of
@@ -1331,6 +1350,7 @@ hipError_t launch_pass16b(const PassArgs &a, int nwg, hipStream_t st)
     const int ts = pass16b_plan(a, &mt, &nb);
     if (ts == 0) return hipErrorInvalidValue;
     const size_t lds = pass16b_lds_bytes(a, ts);
+    if (a.na == 0) return launch_pass16b_noproj(a, ts, lds, nwg, st);
 #define ORTH_P16B(M)                                                                 \
     case M:                                                                          \
         return nb == 2 ? launch_pass16b_f<M, 2>(a, ts, lds, nwg, st)                \
