@@ -13,6 +13,8 @@
 #include <math.h>
 #include <stdlib.h>
 
+#include <type_traits>
+
 #include "orth_device.h"
 
 namespace orth {
@@ -1016,7 +1018,9 @@ __device__ __forceinline__ dbl4 zsteps(const double *sm, int ao, int bo, int LD,
 
 // EARLY: passes without a projection issue the next tile's loads as soon as
 // the current tile is in LDS (nothing else would hide their latency).
-template <int MT, int NB, int F, bool EARLY>
+// V2: every thread loads two consecutive rows with one 16-byte load (needs
+// an even leading dimension and a 16-byte aligned slab).
+template <int MT, int NB, int F, bool EARLY, bool V2>
 __global__ __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_pass16b(PassArgs a)
 {
     constexpr int N1 = 32, TS = 32 * F, LD = TS + 1;
@@ -1069,25 +1073,52 @@ __global__ __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(4, 4))) void
     // Thread tid moves row (tid % TS) of columns tid / TS + i (512 / TS).
     constexpr int PSTEP = kB / TS;
     const int prow = tid & (TS - 1), pcol = tid >> LTS;
-    const int nld = (a.ncols - pcol + PSTEP - 1) / PSTEP;
-    const double *px = a.X + prow + (int64_t)pcol * a.ldX;
-    const int64_t pxs = (int64_t)PSTEP * a.ldX;
-    const int pl = pcol * LD + prow;
-    double pf[kBPF];
+    // Loads: thread tid moves rows (2 (tid % (TS/2)), +1) of columns
+    // tid / (TS/2) + i (1024 / TS) when V2, else row tid % TS of columns
+    // tid / TS + i (512 / TS).
+    typedef double dbl2v __attribute__((ext_vector_type(2)));
+    constexpr int VR = V2 ? 2 : 1;               // rows per thread
+    constexpr int LSTEP = VR * kB / TS;          // columns per load step
+    constexpr int NPF = kBPF / VR;               // load steps
+    const int lrow = VR * (tid & (TS / VR - 1)), lcol = tid / (TS / VR);
+    const int lnld = (a.ncols - lcol + LSTEP - 1) / LSTEP;
+    const double *lx = a.X + lrow + (int64_t)lcol * a.ldX;
+    const int64_t lxs = (int64_t)LSTEP * a.ldX;
+    const int lpl = lcol * LD + lrow;
+    typename std::conditional<V2, dbl2v, double>::type pf[NPF];
     auto load = [&](int64_t r0) {
-        const bool ok = r0 + prow < re;
-        const double *q = px + r0;
+        const double *q = lx + r0;
+        if constexpr (V2) {
+            const bool ok2 = r0 + lrow + 1 < re, ok1 = r0 + lrow < re;
 #pragma unroll
-        for (int i = 0; i < kBPF; i++)
-            if (i < nld) pf[i] = ok ? __builtin_nontemporal_load(q + i * pxs) : 0.0;
+            for (int i = 0; i < NPF; i++)
+                if (i < lnld) {
+                    if (ok2)
+                        pf[i] = __builtin_nontemporal_load((const dbl2v *)(q + i * lxs));
+                    else
+                        pf[i] = dbl2v{ok1 ? q[i * lxs] : 0.0, 0.0};
+                }
+        } else {
+            const bool ok = r0 + lrow < re;
+#pragma unroll
+            for (int i = 0; i < NPF; i++)
+                if (i < lnld) pf[i] = ok ? __builtin_nontemporal_load(q + i * lxs) : 0.0;
+        }
     };
     if (rb < re) load(rb);
 
     for (int64_t r0 = rb; r0 < re; r0 += TS) {
         __syncthreads();
 #pragma unroll
-        for (int i = 0; i < kBPF; i++)
-            if (i < nld) sm[pl + i * PSTEP * LD] = pf[i];
+        for (int i = 0; i < NPF; i++)
+            if (i < lnld) {
+                if constexpr (V2) {
+                    sm[lpl + i * LSTEP * LD] = pf[i].x;
+                    sm[lpl + 1 + i * LSTEP * LD] = pf[i].y;
+                } else {
+                    sm[lpl + i * LSTEP * LD] = pf[i];
+                }
+            }
         __syncthreads();
         if constexpr (EARLY) {
             if (r0 + TS < re) load(r0 + TS);
@@ -1274,19 +1305,29 @@ __global__ __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(4, 4))) void
     }
 }
 
-template <int MT, int NB, int F, bool EARLY = false>
-hipError_t launch_pass16b_t(const PassArgs &a, size_t lds, int nwg, hipStream_t st)
+template <int MT, int NB, int F, bool EARLY, bool V2>
+hipError_t launch_pass16b_v(const PassArgs &a, size_t lds, int nwg, hipStream_t st)
 {
     static bool attr_set = false;
     if (!attr_set) {
-        hipError_t e = hipFuncSetAttribute((const void *)k_pass16b<MT, NB, F, EARLY>,
+        hipError_t e = hipFuncSetAttribute((const void *)k_pass16b<MT, NB, F, EARLY, V2>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize,
                                            (int)(160 * 1024));
         if (e != hipSuccess) return e;
         attr_set = true;
     }
-    k_pass16b<MT, NB, F, EARLY><<<dim3(nwg), dim3(kB), lds, st>>>(a);
+    k_pass16b<MT, NB, F, EARLY, V2><<<dim3(nwg), dim3(kB), lds, st>>>(a);
     return hipGetLastError();
+}
+
+// 16-byte loads when the slab allows them: even leading dimension and a
+// 16-byte aligned base (rows_per and the first row of every chunk are even).
+template <int MT, int NB, int F, bool EARLY = false>
+hipError_t launch_pass16b_t(const PassArgs &a, size_t lds, int nwg, hipStream_t st)
+{
+    const bool v2 = (a.ldX % 2 == 0) && ((uintptr_t)a.X % 16 == 0) && (a.rows_per % 2 == 0);
+    return v2 ? launch_pass16b_v<MT, NB, F, EARLY, true>(a, lds, nwg, st)
+              : launch_pass16b_v<MT, NB, F, EARLY, false>(a, lds, nwg, st);
 }
 
 // Passes without a projection.

@@ -259,3 +259,28 @@ def test_glued_wide(variant):
     assert st["info_block"] == info_o[0]
     if info_o == (0, 0):
         assert within10(I.loo(Qg), I.loo(Qo), 1e-13)
+
+
+@pytest.mark.skipif(torch is None, reason="torch missing")
+@pytest.mark.parametrize("variant", ["bcgsi2P1s", "bcgsi2"])
+@pytest.mark.parametrize("m,ld", [(4097, 4098), (4097, 4097), (6001, 6004)])
+def test_padded_leading_dimension(variant, m, ld):
+    """Odd row counts inside an even leading dimension (two-row loads with a
+    last row that has no partner) and odd leading dimensions (one-row loads)."""
+    n, s = 48, 16
+    X = I.svd_matrix(m, n, 1e3, seed=34)
+    say("case padded ld", variant, m, ld)
+    Xp = np.zeros((ld, n))
+    Xp[:m] = X
+    Xd = torch.from_numpy(np.ascontiguousarray(Xp.T)).cuda()
+    Rd = torch.zeros(n * n, dtype=torch.float64, device="cuda")
+    B.bcgs_device(variant, m, n, s, Xd.data_ptr(), ld, Rd.data_ptr(), n)
+    B.sync()
+    torch.cuda.synchronize()
+    Qp = Xd.cpu().numpy().T
+    R = Rd.cpu().numpy().reshape((n, n), order="F")
+    assert np.array_equal(Qp[m:], Xp[m:])  # rows past m untouched
+    Qo, Ro, info = O.bcgs(variant, X, s)
+    assert info == (0, 0)
+    assert np.abs(R - Ro).max() <= 1e-11 * np.abs(Ro).max()
+    assert np.abs(Qp[:m] - Qo).max() <= 1e-10
