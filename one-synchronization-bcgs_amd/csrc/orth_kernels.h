@@ -12,13 +12,15 @@ namespace orth {
 
 constexpr int kMaxS = 32;  // widest block the kernels support
 constexpr int kLdT = 32;   // leading dimension of every s x s factor
-// Each factor slot holds the 32 x 32 factor (ld kLdT), the reciprocals of
-// its diagonal, and the inverses of its 4 x 4 diagonal blocks (block b at
-// offset kSlotDiag + 16 b, column-major, zero past the factor's width), all
-// written by the Cholesky step.
+// Each factor slot holds the 32 x 32 upper factor T (ld kLdT), the
+// reciprocals of its diagonal, and the row blocks of the blocked solve
+// X T^{-1}: for block b (columns 4b..4b+3), W_b (4 x 32, row-major, at
+// kSlotW + 128 b) holds D_b = inv(T(4b:4b+4, 4b:4b+4)) in columns 4b..4b+3,
+// -D_b T(4b:4b+4, c) in the columns c right of the block, zero elsewhere and
+// past the factor's width.  All are written by the Cholesky step.
 constexpr int kSlotRecip = kLdT * kLdT;
-constexpr int kSlotDiag = kSlotRecip + kLdT;
-constexpr int kSlot = kSlotDiag + (kMaxS / 4) * 16;
+constexpr int kSlotW = kSlotRecip + kLdT;
+constexpr int kSlot = kSlotW + (kMaxS / 4) * 4 * kLdT;
 
 // One row-local block update of a streaming pass:
 //   X(:, col0:col0+w) <- (X(:, col0:col0+w) - X(:, 0:c) * W) * T^{-1}

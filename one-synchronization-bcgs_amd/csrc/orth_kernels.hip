@@ -326,9 +326,9 @@ __device__ int d_chol(int w, const double *G, int64_t ldg, double shift, double 
         T[kSlotRecip + i] = i < w ? 1.0 / sh[i][i] : 0.0;
     // Inverses of the 4 x 4 diagonal blocks, one row per thread:
     // D(i,i) = 1/u(i,i), D(i,j) = -(sum_{k=i}^{j-1} D(i,k) u(k,j)) / u(j,j).
+    __shared__ double dsh[kMaxS][4];
     for (int e = threadIdx.x; e < kMaxS; e += blockDim.x) {
         const int b = e >> 2, i = e & 3, o = 4 * b;
-        double *D = T + kSlotDiag + 16 * b;
         double d[4] = {0.0, 0.0, 0.0, 0.0};
         if (o + i < w) {
             d[i] = 1.0 / sh[o + i][o + i];
@@ -338,7 +338,21 @@ __device__ int d_chol(int w, const double *G, int64_t ldg, double shift, double 
                 d[j] = -v * (1.0 / sh[o + j][o + j]);
             }
         }
-        for (int j = 0; j < 4; j++) D[i + 4 * j] = d[j];
+        for (int j = 0; j < 4; j++) dsh[e][j] = d[j];
+    }
+    __syncthreads();
+    // Row blocks W_b of the blocked solve (see orth_kernels.h).
+    for (int e = threadIdx.x; e < (kMaxS / 4) * 4 * kLdT; e += blockDim.x) {
+        const int b = e / (4 * kLdT), k = (e / kLdT) & 3, c = e % kLdT, o = 4 * b;
+        double v = 0.0;
+        if (o + k < w && c < w) {
+            if (c >= o && c < o + 4) {
+                v = dsh[o + k][c - o];
+            } else if (c >= o + 4) {
+                for (int j = 0; j < 4 && o + j < w; j++) v = fma(-dsh[o + k][j], sh[o + j][c], v);
+            }
+        }
+        T[kSlotW + e] = v;
     }
     const int info = info_s;
     __syncthreads();
@@ -1058,12 +1072,9 @@ __global__ __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(4, 4))) void
     // the columns right of block b.
     const int gq = wave & 1;
     const double *Tq = gq < a.ngrp ? ORTH_G(a, gq, T) : nullptr;
-    double bd[4], bt[3];
+    double bw[4];
 #pragma unroll
-    for (int b = 0; b < 4; b++) {
-        bd[b] = (Tq && (li >> 2) == b) ? Tq[kSlotDiag + 16 * b + lk + 4 * (li & 3)] : 0.0;
-        if (b < 3) bt[b] = (Tq && li >= 4 * b + 4) ? -Tq[4 * b + lk + li * kLdT] : 0.0;
-    }
+    for (int b = 0; b < 4; b++) bw[b] = Tq ? Tq[kSlotW + 128 * b + lk * kLdT + li] : 0.0;
     dbl4 acc[MT];
 #pragma unroll
     for (int j = 0; j < MT; j++) acc[j] = dbl4{0.0, 0.0, 0.0, 0.0};
@@ -1209,11 +1220,12 @@ __global__ __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(4, 4))) void
                 __syncthreads();
                 // Triangular solve X <- X T^{-1} of each group, blocked by
                 // four columns on FP64 MFMA (the inverted-diagonal-block
-                // scheme of blocked dtrsm): x_b = b_b D_b, then the columns
-                // right of block b lose x_b T(b, >).  Wave w takes group
-                // (w & 1) and 16-row chunks (w >> 1) + 4 i; the chunk lives
-                // in MFMA accumulator layout (lane: column li, rows lk + 4 v)
-                // and each block goes through LDS once to become an A operand.
+                // scheme of blocked dtrsm): block b's columns become
+                // b_b D_b and the columns right of it lose b_b D_b T(b, >),
+                // one product with W_b per block.  Wave w takes group (w & 1)
+                // and 16-row chunks (w >> 1) + 4 i; the chunk lives in MFMA
+                // accumulator layout (lane: column li, rows lk + 4 v) and each
+                // block goes through LDS once to become an A operand.
                 if (Tq && gq >= glo && gq < ghi) {
                     const int t0 = ORTH_G(a, gq, t0), w = ORTH_G(a, gq, w);
                     for (int ch = wave >> 1; ch < TS / 16; ch += 4) {
@@ -1227,7 +1239,7 @@ __global__ __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(4, 4))) void
                             // Columns at or past the group width keep their
                             // tile contents (they only feed each other).
                             const bool mineb = (li >> 2) == b && li < w;
-                            if (mineb) {
+                            if (mineb && b > 0) {
 #pragma unroll
                                 for (int v = 0; v < 4; v++) sm[base + 4 * v] = x[v];
                             }
@@ -1238,16 +1250,7 @@ __global__ __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(4, 4))) void
                             __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
                             const double av = sm[abase + 4 * b * LD];
                             if ((li >> 2) == b) x = dbl4{0.0, 0.0, 0.0, 0.0};
-                            x = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bd[b], x, 0, 0, 0);
-                            if (b < 3) {
-                                if (mineb) {
-#pragma unroll
-                                    for (int v = 0; v < 4; v++) sm[base + 4 * v] = x[v];
-                                }
-                                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-                                const double xv = sm[abase + 4 * b * LD];
-                                x = __builtin_amdgcn_mfma_f64_16x16x4f64(xv, bt[b], x, 0, 0, 0);
-                            }
+                            x = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bw[b], x, 0, 0, 0);
                         }
                         if (li < w) {
 #pragma unroll

@@ -146,12 +146,9 @@ __global__ __launch_bounds__(kM) __attribute__((amdgpu_waves_per_eu(MX == kMxFP6
     // Triangular-solve operands of this wave's group (as in the FP64 pass).
     const int gq = wave & 1;
     const double *Tq = gq < a.ngrp ? ORTH_G(a, gq, T) : nullptr;
-    double bd[4], bt[3];
+    double bw[4];
 #pragma unroll
-    for (int b = 0; b < 4; b++) {
-        bd[b] = (Tq && (li >> 2) == b) ? Tq[kSlotDiag + 16 * b + lk + 4 * (li & 3)] : 0.0;
-        if (b < 3) bt[b] = (Tq && li >= 4 * b + 4) ? -Tq[4 * b + lk + li * kLdT] : 0.0;
-    }
+    for (int b = 0; b < 4; b++) bw[b] = Tq ? Tq[kSlotW + 128 * b + lk * kLdT + li] : 0.0;
 
     const int64_t rb = (int64_t)blockIdx.x * a.rows_per;
     const int64_t re = min(a.m, rb + a.rows_per);
@@ -242,8 +239,8 @@ __global__ __launch_bounds__(kM) __attribute__((amdgpu_waves_per_eu(MX == kMxFP6
                 }
                 __syncthreads();
                 // Triangular solve X <- X T^{-1}, blocked by four columns on
-                // FP64 MFMA with the inverted diagonal blocks (see the FP64
-                // pass); wave w takes group (w & 1) and 16-row chunk w >> 1.
+                // FP64 MFMA with the W_b row blocks (see the FP64 pass); wave
+                // w takes group (w & 1) and 16-row chunk w >> 1.
                 if (Tq && gq >= glo && gq < ghi && (wave >> 1) < TS / 16) {
                     const int t0 = ORTH_G(a, gq, t0), w = ORTH_G(a, gq, w);
                     const int ch = wave >> 1;
@@ -255,23 +252,14 @@ __global__ __launch_bounds__(kM) __attribute__((amdgpu_waves_per_eu(MX == kMxFP6
 #pragma unroll
                     for (int b = 0; b < 4; b++) {
                         const bool mineb = (li >> 2) == b && li < w;
-                        if (mineb) {
+                        if (mineb && b > 0) {
 #pragma unroll
                             for (int v = 0; v < 4; v++) sm[base + 4 * v] = x[v];
                         }
                         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
                         const double av = sm[abase + 4 * b * LD];
                         if ((li >> 2) == b) x = dbl4{0.0, 0.0, 0.0, 0.0};
-                        x = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bd[b], x, 0, 0, 0);
-                        if (b < 3) {
-                            if (mineb) {
-#pragma unroll
-                                for (int v = 0; v < 4; v++) sm[base + 4 * v] = x[v];
-                            }
-                            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-                            const double xv = sm[abase + 4 * b * LD];
-                            x = __builtin_amdgcn_mfma_f64_16x16x4f64(xv, bt[b], x, 0, 0, 0);
-                        }
+                        x = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bw[b], x, 0, 0, 0);
                     }
                     if (li < w) {
 #pragma unroll
