@@ -248,6 +248,20 @@ def test_alternate_pass_kernels(variant, env, monkeypatch):
     check_against_oracle(variant, X, 16)
 
 
+@pytest.mark.parametrize("variant", VARIANTS)
+@pytest.mark.parametrize("m", [65538, 9906, 96])
+def test_tile_boundary_shapes(variant, m):
+    """Several super-tiles per workgroup, a partial last super-tile, and a
+    slab shorter than one super-tile, at n = 272 (the widest tile, 32 rows)
+    and n = 48 (256-row super-tiles)."""
+    for n in (272, 48):
+        if m < n:
+            continue
+        say("case tile boundary", variant, m, n)
+        X = I.svd_matrix(m, n, 1e3, seed=35)
+        check_against_oracle(variant, X, 16)
+
+
 @pytest.mark.parametrize("variant", ["bcgsi2P1s", "bcgsi2", "bcgsi2P2s", "bcgspipi2"])
 def test_glued_wide(variant):
     """Glued blocks at n = 512, s = 32: breakdown reported on the same block
