@@ -16,7 +16,7 @@ LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "liborth_hip.so")
 
 SOURCES = ["orth_kernels.hip", "orth_mixed.hip", "orth_host.cpp"]
-HEADERS = ["orth_kernels.h", "orth_device.h"]
+HEADERS = ["orth_kernels.h", "orth_device.h", "orth_pass16b.h"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "orth_kernels.h"
 
 namespace orth {
@@ -57,6 +59,97 @@ __device__ __forceinline__ dbl4 zsteps_g(const double *sm, const double *__restr
         z = __builtin_amdgcn_mfma_f64_16x16x4f64(sm[ao + 4 * ks * LD], C[bo + 4 * ks * N1], z, 0,
                                                  0, 0);
     return z;
+}
+
+// ---- reduced-precision MFMA operands (mixed form)
+
+typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
+typedef __attribute__((ext_vector_type(8))) _Float16 f16x8;
+typedef __attribute__((ext_vector_type(4))) float f32x4;
+
+// Reduced-precision MFMA operand: 8 values of one lane (k = 8 (l / 16) + e),
+// as a high part and, for the split form, the low part FP32(x) - FP32(hi).
+template <int MX>
+struct LpFrag {
+    typedef typename std::conditional<MX == kMxFP16, f16x8, bf16x8>::type V;
+    V hi, lo;
+};
+
+// Load 8 FP64 values p[0], p[st], ..., p[7 st] from LDS and convert them,
+// four at a time (bounded registers).
+template <int MX>
+__device__ __forceinline__ LpFrag<MX> lp_frag(const double *p, int st)
+{
+    LpFrag<MX> f;
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        float x[4];
+#pragma unroll
+        for (int e = 0; e < 4; e++) x[e] = (float)p[(4 * h + e) * st];
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            if constexpr (MX == kMxFP16) {
+                f.hi[4 * h + e] = (_Float16)x[e];
+            } else {
+                f.hi[4 * h + e] = (__bf16)x[e];
+                if constexpr (MX == kMxBF16X3) f.lo[4 * h + e] = (__bf16)(x[e] - (float)f.hi[4 * h + e]);
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    return f;
+}
+
+// The same from global memory with a validity mask (coefficients past k1).
+template <int MX>
+__device__ __forceinline__ LpFrag<MX> lp_frag_g(const double *p, int st, int nvalid)
+{
+    LpFrag<MX> f;
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        float x[4];
+#pragma unroll
+        for (int e = 0; e < 4; e++) x[e] = 4 * h + e < nvalid ? (float)p[(4 * h + e) * st] : 0.0f;
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            if constexpr (MX == kMxFP16) {
+                f.hi[4 * h + e] = (_Float16)x[e];
+            } else {
+                f.hi[4 * h + e] = (__bf16)x[e];
+                if constexpr (MX == kMxBF16X3) f.lo[4 * h + e] = (__bf16)(x[e] - (float)f.hi[4 * h + e]);
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    return f;
+}
+
+// One 16 x 16 x 32 product in the reduced precision, accumulated into t
+// (FP32): lane l holds rows 4 (l / 16) + v of column l % 16 of the result.
+template <int MX>
+__device__ __forceinline__ f32x4 lp_mma(const LpFrag<MX> &a, const LpFrag<MX> &b, f32x4 t)
+{
+    if constexpr (MX == kMxFP16) {
+        t = __builtin_amdgcn_mfma_f32_16x16x32_f16(a.hi, b.hi, t, 0, 0, 0);
+    } else {
+        if constexpr (MX == kMxBF16X3) {
+            t = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.hi, b.lo, t, 0, 0, 0);
+            t = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.lo, b.hi, t, 0, 0, 0);
+        }
+        t = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.hi, b.hi, t, 0, 0, 0);
+    }
+    return t;
+}
+
+// Is Z tile nt consumed by a group whose product is marked reduced-precision?
+__device__ __forceinline__ bool ztile_lp(const PassArgs &a, int nt)
+{
+    bool u = false;
+    for (int g = 0; g < a.ngrp; g++) {
+        const int z0 = ORTH_G(a, g, zoff), z1 = z0 + ORTH_G(a, g, w);
+        u = u || (ORTH_G(a, g, lp) && nt * 16 < z1 && nt * 16 + 16 > z0);
+    }
+    return u;
 }
 
 }  // namespace orth

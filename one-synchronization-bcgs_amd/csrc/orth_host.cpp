@@ -526,6 +526,7 @@ class Run {
         const int ts16 =
             (S_ == 16 && !mx_ && !getenv("ORTH_PASS256")) ? pass16b_plan(pa, &mt, &nbt) : 0;
         int64_t unit = kPassRows;
+        int tsm = 0;
         if (mx_) {
             // Columns at or past the written frontier still live only in X32.
             pa.X32 = x32_ + cb * ldx32_;
@@ -533,7 +534,13 @@ class Run {
             pa.raw0 = (int)std::max<int64_t>(0, frontier_ - cb);
             if (pa.raw0 < pa.ncols && pa.raw0 % 16 != 0) return nullptr;
             pa.lpb = lp_from_ >= 0 ? (int)(lp_from_ - cb) : (1 << 30);
-            if (pass16m_lds_bytes(pa) == 0) return nullptr;
+            // The 512-thread pass with super-tiles when the slab allows it,
+            // else the 32-row mixed pass.
+            tsm = getenv("ORTH_MIXED_PASS32") ? 0 : pass16bm_plan(pa);
+            if (tsm > 0)
+                unit = tsm;
+            else if (pass16m_lds_bytes(pa) == 0)
+                return nullptr;
         } else if (ts16 > 0) {
             unit = ts16;
         } else if (S_ == 16 && na > 0) {
@@ -559,7 +566,9 @@ class Run {
             wcols += pend_.g[g].w;
         }
         if (!timed(c_, ORTH_K_PASS, st_, flops, 8.0 * m_ * (ncols + wcols), [&] {
-                if (mx_) return launch_pass16m(pa, mx_, nwg, st_);
+                if (mx_)
+                    return tsm > 0 ? launch_pass16bm(pa, mx_, nwg, st_)
+                                   : launch_pass16m(pa, mx_, nwg, st_);
                 return ts16 > 0 ? launch_pass16b(pa, nwg, st_) : launch_pass(S_, pa, nwg, st_);
             }))
             return nullptr;

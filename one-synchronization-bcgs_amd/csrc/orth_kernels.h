@@ -156,6 +156,7 @@ hipError_t launch_pass(int S, const PassArgs &a, int nwg, hipStream_t st);
 // pass (multiple of 32, 0 when it does not apply); rows_per must be a
 // multiple of it.
 int pass16b_plan(const PassArgs &a, int *mt, int *nb);
+size_t pass16b_lds_bytes(const PassArgs &a, int ts);
 hipError_t launch_pass16b(const PassArgs &a, int nwg, hipStream_t st);
 
 // Mixed-precision s = 16 pass (32-row tiles, 512 threads): mx selects the
@@ -165,6 +166,11 @@ hipError_t launch_pass16b(const PassArgs &a, int nwg, hipStream_t st);
 enum { kMxBF16 = 1, kMxFP16 = 2, kMxBF16X3 = 3, kMxFP64 = 4 };
 size_t pass16m_lds_bytes(const PassArgs &a);
 hipError_t launch_pass16m(const PassArgs &a, int mx, int nwg, hipStream_t st);
+
+// The mixed form on the 512-thread s = 16 pass (super-tiles, two workgroups
+// per CU): 0 when the shape does not fit it (then launch_pass16m applies).
+int pass16bm_plan(const PassArgs &a);
+hipError_t launch_pass16bm(const PassArgs &a, int mx, int nwg, hipStream_t st);
 
 // Zero n entries with a kernel (no memset nodes, so the sequence can be
 // captured in a graph).

@@ -20,100 +20,12 @@
 #include <type_traits>
 
 #include "orth_device.h"
+#include "orth_pass16b.h"
 
 namespace orth {
 
-typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
-typedef __attribute__((ext_vector_type(8))) _Float16 f16x8;
-typedef __attribute__((ext_vector_type(4))) float f32x4;
-
 constexpr int kM = 512;    // threads per workgroup
 constexpr int kMPF = 16;   // prefetch registers per thread (32 rows x 256 columns)
-
-// Reduced-precision MFMA operand: 8 values of one lane (k = 8 (l / 16) + e),
-// as a high part and, for the split form, the low part FP32(x) - FP32(hi).
-template <int MX>
-struct LpFrag {
-    typedef typename std::conditional<MX == kMxFP16, f16x8, bf16x8>::type V;
-    V hi, lo;
-};
-
-// Load 8 FP64 values p[0], p[st], ..., p[7 st] from LDS and convert them,
-// four at a time (bounded registers).
-template <int MX>
-__device__ __forceinline__ LpFrag<MX> lp_frag(const double *p, int st)
-{
-    LpFrag<MX> f;
-#pragma unroll
-    for (int h = 0; h < 2; h++) {
-        float x[4];
-#pragma unroll
-        for (int e = 0; e < 4; e++) x[e] = (float)p[(4 * h + e) * st];
-#pragma unroll
-        for (int e = 0; e < 4; e++) {
-            if constexpr (MX == kMxFP16) {
-                f.hi[4 * h + e] = (_Float16)x[e];
-            } else {
-                f.hi[4 * h + e] = (__bf16)x[e];
-                if constexpr (MX == kMxBF16X3) f.lo[4 * h + e] = (__bf16)(x[e] - (float)f.hi[4 * h + e]);
-            }
-        }
-        __builtin_amdgcn_sched_barrier(0);
-    }
-    return f;
-}
-
-// The same from global memory with a validity mask (coefficients past k1).
-template <int MX>
-__device__ __forceinline__ LpFrag<MX> lp_frag_g(const double *p, int st, int nvalid)
-{
-    LpFrag<MX> f;
-#pragma unroll
-    for (int h = 0; h < 2; h++) {
-        float x[4];
-#pragma unroll
-        for (int e = 0; e < 4; e++) x[e] = 4 * h + e < nvalid ? (float)p[(4 * h + e) * st] : 0.0f;
-#pragma unroll
-        for (int e = 0; e < 4; e++) {
-            if constexpr (MX == kMxFP16) {
-                f.hi[4 * h + e] = (_Float16)x[e];
-            } else {
-                f.hi[4 * h + e] = (__bf16)x[e];
-                if constexpr (MX == kMxBF16X3) f.lo[4 * h + e] = (__bf16)(x[e] - (float)f.hi[4 * h + e]);
-            }
-        }
-        __builtin_amdgcn_sched_barrier(0);
-    }
-    return f;
-}
-
-// One 16 x 16 x 32 product in the reduced precision, accumulated into t
-// (FP32): lane l holds rows 4 (l / 16) + v of column l % 16 of the result.
-template <int MX>
-__device__ __forceinline__ f32x4 lp_mma(const LpFrag<MX> &a, const LpFrag<MX> &b, f32x4 t)
-{
-    if constexpr (MX == kMxFP16) {
-        t = __builtin_amdgcn_mfma_f32_16x16x32_f16(a.hi, b.hi, t, 0, 0, 0);
-    } else {
-        if constexpr (MX == kMxBF16X3) {
-            t = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.hi, b.lo, t, 0, 0, 0);
-            t = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.lo, b.hi, t, 0, 0, 0);
-        }
-        t = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.hi, b.hi, t, 0, 0, 0);
-    }
-    return t;
-}
-
-// Is Z tile nt consumed by a group whose product is marked reduced-precision?
-__device__ __forceinline__ bool ztile_lp(const PassArgs &a, int nt)
-{
-    bool u = false;
-    for (int g = 0; g < a.ngrp; g++) {
-        const int z0 = ORTH_G(a, g, zoff), z1 = z0 + ORTH_G(a, g, w);
-        u = u || (ORTH_G(a, g, lp) && nt * 16 < z1 && nt * 16 + 16 > z0);
-    }
-    return u;
-}
 
 template <int NB, int MX>
 __global__ __launch_bounds__(kM) __attribute__((amdgpu_waves_per_eu(MX == kMxFP64 ? 4 : 2, MX == kMxFP64 ? 4 : 2))) void k_pass16m(PassArgs a)
@@ -375,6 +287,81 @@ hipError_t launch_pass16m(const PassArgs &a, int mx, int nwg, hipStream_t st)
     case kMxFP16: return launch_pass16m_mx<kMxFP16>(a, lds, nwg, st);
     case kMxBF16X3: return launch_pass16m_mx<kMxBF16X3>(a, lds, nwg, st);
     case kMxFP64: return launch_pass16m_mx<kMxFP64>(a, lds, nwg, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// The mixed form on the 512-thread pass (k_pass16b with MX = the precision).
+// ---------------------------------------------------------------------------
+
+int pass16bm_plan(const PassArgs &a)
+{
+    int mt = 0, nb = 0;
+    const int ts = pass16b_plan(a, &mt, &nb);
+    if (ts == 0 || !a.X32) return 0;
+    // 16-byte loads of the FP64 columns (the instantiated form).
+    if (a.ldX % 2 != 0 || (uintptr_t)a.X % 16 != 0) return 0;
+    // Reduced-precision coefficient chunks held per wave.
+    const int nch = (a.k1 + 31) >> 5;
+    if (ts == 32 ? nch - (nch >> 1) > kLpC : nch > kLpC) return 0;
+    if (a.raw0 < a.ncols && a.raw0 % 16 != 0) return 0;
+    return ts;
+}
+
+template <int MT, int NB, int F, bool EARLY, int MX>
+hipError_t launch_bm_t(const PassArgs &a, size_t lds, int nwg, hipStream_t st)
+{
+    return launch_pass16b_v<MT, NB, F, EARLY, true, MX>(a, lds, nwg, st);
+}
+
+template <int MX>
+hipError_t launch_bm_mx(const PassArgs &a, int ts, int mt, int nb, size_t lds, int nwg,
+                        hipStream_t st)
+{
+    if (a.na == 0) {
+        switch (ts) {
+        case 32: return launch_bm_t<1, 1, 1, true, MX>(a, lds, nwg, st);
+        case 64: return launch_bm_t<1, 1, 2, true, MX>(a, lds, nwg, st);
+        case 128: return launch_bm_t<1, 1, 4, true, MX>(a, lds, nwg, st);
+        case 256: return launch_bm_t<1, 1, 8, true, MX>(a, lds, nwg, st);
+        default: return hipErrorInvalidValue;
+        }
+    }
+#define ORTH_BM(M, F)                                                                  \
+    if (mt == M)                                                                       \
+        return nb == 2 ? launch_bm_t<M, 2, F, false, MX>(a, lds, nwg, st)              \
+                       : launch_bm_t<M, 1, F, false, MX>(a, lds, nwg, st);
+    switch (ts) {
+    case 32:
+        ORTH_BM(1, 1) ORTH_BM(2, 1) ORTH_BM(3, 1) ORTH_BM(4, 1)
+        break;
+    case 64:
+        ORTH_BM(1, 2) ORTH_BM(2, 2)
+        break;
+    case 128:
+        ORTH_BM(1, 4)
+        break;
+    case 256:
+        ORTH_BM(1, 8)
+        break;
+    default: break;
+    }
+#undef ORTH_BM
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_pass16bm(const PassArgs &a, int mx, int nwg, hipStream_t st)
+{
+    int mt = 0, nb = 0;
+    const int ts = pass16bm_plan(a) ? pass16b_plan(a, &mt, &nb) : 0;
+    if (ts == 0 || a.rows_per % ts != 0) return hipErrorInvalidValue;
+    const size_t lds = pass16b_lds_bytes(a, ts);
+    switch (mx) {
+    case kMxBF16: return launch_bm_mx<kMxBF16>(a, ts, mt, nb, lds, nwg, st);
+    case kMxFP16: return launch_bm_mx<kMxFP16>(a, ts, mt, nb, lds, nwg, st);
+    case kMxBF16X3: return launch_bm_mx<kMxBF16X3>(a, ts, mt, nb, lds, nwg, st);
+    case kMxFP64: return launch_bm_mx<kMxFP64>(a, ts, mt, nb, lds, nwg, st);
     default: return hipErrorInvalidValue;
     }
 }

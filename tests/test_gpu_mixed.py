@@ -44,11 +44,23 @@ def mixed(X, s, precision):
     return Q, R, st, X32.astype(np.float64)
 
 
-@pytest.mark.parametrize("shape", [(20000, 128), (8192, 64), (5000, 80)])
-def test_fp64_products_match_oracle(shape):
+KERNELS = ["pass512", "pass32"]
+
+
+def pick(kernel, monkeypatch):
+    """pass512: the 512-thread pass with super-tiles (default); pass32: the
+    32-row mixed pass (also the fallback for odd leading dimensions)."""
+    if kernel == "pass32":
+        monkeypatch.setenv("ORTH_MIXED_PASS32", "1")
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+@pytest.mark.parametrize("shape", [(20000, 128), (8192, 64), (5000, 80), (5001, 48), (300, 256)])
+def test_fp64_products_match_oracle(shape, kernel, monkeypatch):
+    pick(kernel, monkeypatch)
     m, n = shape
     X = I.svd_matrix(m, n, 1e4, seed=41)
-    say("case mixed fp64", shape)
+    say("case mixed fp64", shape, kernel)
     Q, R, st, Xr = mixed(X, 16, "fp64")
     Qo, Ro, info = O.bcgs("bcgsi2P1s", Xr, 16)
     assert st["hip_error"] == 0 and st["info_block"] == 0 and info == (0, 0)
@@ -60,11 +72,13 @@ def test_fp64_products_match_oracle(shape):
 LEVEL = {"bf16": 2.0 ** -9, "fp16": 2.0 ** -12, "bf16x3": 2.0 ** -18}
 
 
+@pytest.mark.parametrize("kernel", KERNELS)
 @pytest.mark.parametrize("precision", ["bf16", "fp16", "bf16x3"])
-def test_reduced_precision_first_pass(precision):
+def test_reduced_precision_first_pass(precision, kernel, monkeypatch):
+    pick(kernel, monkeypatch)
     m, n = 65536, 128
     X = I.svd_matrix(m, n, 1e2, seed=42)
-    say("case mixed", precision)
+    say("case mixed", precision, kernel)
     Q, R, st, Xr = mixed(X, 16, precision)
     assert st["hip_error"] == 0 and st["info_block"] == 0
     Qo, Ro, _ = O.bcgs("bcgsi2P1s", Xr, 16)
