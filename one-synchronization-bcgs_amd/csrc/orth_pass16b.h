@@ -140,7 +140,11 @@ __global__ __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(
 #pragma unroll
             for (int i = 0; i < NPF; i++)
                 if (i < lnld) {
-                    if (MX != 0 && lcol + i * LSTEP >= a.raw0) {
+                    // Raw columns: a.raw0 is a multiple of 16 and a wave's
+                    // lanes move at most four consecutive columns (a
+                    // 4-aligned group), so the choice is uniform per wave.
+                    if (MX != 0 &&
+                        __builtin_amdgcn_readfirstlane((int)(lcol + i * LSTEP >= a.raw0))) {
                         // Raw column: two FP32 values, converted exactly.
                         const float *q32 = lx32 + r0 + i * lxs32;
                         pf[i] = dbl2v{ok1 ? (double)__builtin_nontemporal_load(q32) : 0.0,
@@ -155,7 +159,8 @@ __global__ __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(
 #pragma unroll
             for (int i = 0; i < NPF; i++)
                 if (i < lnld) {
-                    if (MX != 0 && lcol + i * LSTEP >= a.raw0)
+                    if (MX != 0 &&
+                        __builtin_amdgcn_readfirstlane((int)(lcol + i * LSTEP >= a.raw0)))
                         pf[i] = ok ? (double)__builtin_nontemporal_load(lx32 + r0 + i * lxs32) : 0.0;
                     else
                         pf[i] = ok ? __builtin_nontemporal_load(q + i * lxs) : 0.0;
