@@ -223,6 +223,17 @@ def suite(args):
                   "ms": round(ms, 3), "GBs_alg": round(b_alg(m, n, s) / (ms / 1e3) / 1e9, 1),
                   "loo": loo, "rel_residual": res, "info_block": st["info_block"],
                   "info_pivot": st["info_pivot"]})
+        # The Pythagorean variants again with the optional Cholesky shift
+        # c u tr(G), c = 11 (m s + s (s + 1)).
+        pc = 11.0 * (m * s + s * (s + 1))
+        B.set_pythagorean_shift(pc)
+        for v in ("bcgsi2P1s", "bcgspipi2", "bcgsi21s", "bcgsi2P2s"):
+            ms, st, loo, res = run_one(torch, B, v, X0, m, n, s, reps=1)
+            emit({"config": "glued-shift", "variant": v, "m": m, "n": n, "s": s,
+                  "delta": delta, "shift_c": pc, "ms": round(ms, 3), "loo": loo,
+                  "rel_residual": res, "info_block": st["info_block"],
+                  "info_pivot": st["info_pivot"]})
+        B.set_pythagorean_shift(0.0)
         del X0
         torch.cuda.empty_cache()
     return out

@@ -129,6 +129,17 @@ int orth_update_device(int64_t m, double *dX, int64_t ldX, int64_t col0,
                        int64_t w, int64_t c, const double *dW, int64_t ldW,
                        const double *dT, void *stream);
 
+/* Optional shift of the Pythagorean Cholesky steps (no counterpart in the
+ * reference, which factors G - S^T S as is): later calls on every device
+ * factor G - S^T S + c u tr(G) I instead, u = 2^-53 and G the Gram of the
+ * block before the subtraction (P-1S, A-1S, PIPI+ and the P-2S second
+ * pass; BCGSI+ has no Pythagorean step).  c = 0, the default, restores the
+ * reference's arithmetic.  A shift keeps the factorization defined when
+ * G - S^T S has lost positive definiteness to rounding (nearly dependent
+ * blocks); it does not restore orthogonality there.  Returns 0, or 2 for a
+ * negative or non-finite c. */
+int orth_set_pythagorean_shift(double c);
+
 /* Kernel classes for orth_kernel_times. */
 enum {
     ORTH_K_PROJ = 0,    /* projection partials A^T B (MFMA FP64)       */

@@ -75,6 +75,8 @@ def lib():
     L.orth_update_device.restype = ctypes.c_int
     L.orth_update_device.argtypes = [_i64, _vp, _i64, _i64, _i64, _i64, _vp,
                                      _i64, _vp, _vp]
+    L.orth_set_pythagorean_shift.restype = ctypes.c_int
+    L.orth_set_pythagorean_shift.argtypes = [ctypes.c_double]
     L.orth_kernel_timing.restype = ctypes.c_int
     L.orth_kernel_timing.argtypes = [ctypes.c_int]
     L.orth_kernel_times.restype = ctypes.c_int
@@ -216,6 +218,13 @@ def sync():
 
 
 KERNELS = {"proj": 0, "reduce": 1, "update": 2, "small": 3, "pass": 4}
+
+
+def set_pythagorean_shift(c):
+    """Shift the Pythagorean Cholesky steps of later calls by c u tr(G)
+    (0 restores the reference's arithmetic)."""
+    if lib().orth_set_pythagorean_shift(float(c)) != 0:
+        raise ValueError("shift constant must be finite and >= 0")
 
 
 def kernel_timing(enable):

@@ -15,6 +15,8 @@
 #include <math.h>
 
 #include <algorithm>
+#include <atomic>
+#include <cmath>
 #include <climits>
 #include <mutex>
 #include <vector>
@@ -62,6 +64,9 @@ struct DevCtx {
 };
 
 DevCtx g_ctx[kMaxDevices];
+
+// Shift constant of the Pythagorean Cholesky steps (orth_set_pythagorean_shift).
+std::atomic<double> g_pshift{0.0};
 
 thread_local int tl_hip_error = 0;
 
@@ -221,6 +226,7 @@ class Run {
         a.Ctmp = Ctmp_;
         a.status = c_.status_d;
         a.Tbig = c_.Tbig;
+        a.pshift = g_pshift.load();
         if (fused_) {
             a.Cout = c_.Cbuf;
             a.n1 = n1_;
@@ -1035,6 +1041,11 @@ std::vector<int64_t> full_key(const BatchArgs &a, const BatchCfg &cfg)
     k.push_back(cfg.graph ? 1 : 0);
     k.push_back(cfg.streams);
     k.push_back((int64_t)(cfg.wg_scale * 1e6));
+    // Captured graphs hold the small steps' arguments, the shift among them.
+    const double ps = g_pshift.load();
+    int64_t psb = 0;
+    memcpy(&psb, &ps, sizeof(psb));
+    k.push_back(psb);
     for (int i = 0; i < a.count; i++) {
         k.push_back((int64_t)(intptr_t)a.X[i]);
         k.push_back(a.ldX[i]);
@@ -1492,6 +1503,13 @@ int orth_accuracy_device(int64_t m, int64_t n, const double *dX, int64_t ldX, co
     out[0] = sqrt(loo);
     out[1] = sqrt(sr);
     out[2] = sqrt(sx);
+    return 0;
+}
+
+int orth_set_pythagorean_shift(double c)
+{
+    if (!(c >= 0.0) || !std::isfinite(c)) return 2;
+    g_pshift.store(c);
     return 0;
 }
 

@@ -76,6 +76,9 @@ struct SmallArgs {
     double *Cout;
     int n1;
     int S;  // block template width: column offset of the second Z group
+    // Pythagorean Cholesky steps factor G - S^T S + pshift * u * tr(G) * I
+    // (u = 2^-53, G the Gram before the subtraction); 0 = no shift.
+    double pshift;
 };
 
 hipError_t launch_update(int S, double *X, int64_t ldX, int64_t m,

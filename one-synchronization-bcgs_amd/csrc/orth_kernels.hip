@@ -431,6 +431,22 @@ __device__ void note_info(int *status, int blk, int info)
     }
 }
 
+// Shift of a Pythagorean Cholesky step: pshift * u * tr(G) over the w x w
+// Gram G (ld ldg) before S^T S is subtracted; block-uniform, 0 when off.
+__device__ double d_pyth_shift(const SmallArgs &a, int w, const double *G, int64_t ldg)
+{
+    __shared__ double tr;
+    if (a.pshift == 0.0) return 0.0;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int i = 0; i < w; i++) t += G[i + i * ldg];
+        tr = t;
+    }
+    __syncthreads();
+    return a.pshift * 1.1102230246251565e-16 * tr;
+}
+
 __global__ __launch_bounds__(256) void k_small(SmallArgs a)
 {
     double *P = a.P;
@@ -440,16 +456,18 @@ __global__ __launch_bounds__(256) void k_small(SmallArgs a)
     switch (a.op) {
     case OP_OPEN_PYTH: {
         // G -= S^T S; R(0:c, b) = S; Sigma = chol(G)  (SRC/orth.c:115-119).
+        const double sh = d_pyth_shift(a, wb, P + c, na);
         d_tn(c, wb, P, na, wb, P, na, P + c, na, -1.0, 1.0);
         d_copy(c, wb, P, na, Rb, a.ldR);
-        note_info(a.status, a.blk, d_chol(wb, P + c, na, 0.0, a.Sig));
+        note_info(a.status, a.blk, d_chol(wb, P + c, na, sh, a.Sig));
         if (a.Cout) d_emit(a.Cout, a.n1, a.S, c, c, wb, P, na, 0, nullptr, 0);
         break;
     }
     case OP_CLOSE_OPEN: {
         // Omega = chol(Q^T Q^ - Y^T Y) (SRC/orth.c:134-137, :200-204).
+        const double sho = d_pyth_shift(a, wb, P + c, na);
         d_tn(c, wb, P, na, wb, P, na, P + c, na, -1.0, 1.0);
-        note_info(a.status, a.blk, d_chol(wb, P + c, na, 0.0, a.Om));
+        note_info(a.status, a.blk, d_chol(wb, P + c, na, sho, a.Om));
         if (a.flags & F_1S) {
             // R(0:c, b) = S + Y; R_bb = Omega (SRC/orth.c:356-360).
             for (int e = threadIdx.x; e < c * wb; e += blockDim.x) {
@@ -472,8 +490,9 @@ __global__ __launch_bounds__(256) void k_small(SmallArgs a)
             if (a.flags & F_PYTH_NEXT) {
                 // Sigma_n = chol(X_n^T X_n - S^T S) (SRC/orth.c:160-171).
                 double *H = Pn + c + wb;
+                const double shn = d_pyth_shift(a, wn, H, na);
                 d_tn(c + wb, wn, Pn, na, wn, Pn, na, H, na, -1.0, 1.0);
-                note_info(a.status, a.blk + 1, d_chol(wn, H, na, 0.0, a.Sig));
+                note_info(a.status, a.blk + 1, d_chol(wn, H, na, shn, a.Sig));
             }
         }
         if (a.Cout && (a.flags & F_SEQ)) {
@@ -502,16 +521,18 @@ __global__ __launch_bounds__(256) void k_small(SmallArgs a)
     }
     case OP_PIPI1: {
         // Sigma = chol(X^T X - S^T S); R(0:c, b) = S (SRC/orth.c:425-429).
+        const double sh = d_pyth_shift(a, wb, P + c, na);
         d_tn(c, wb, P, na, wb, P, na, P + c, na, -1.0, 1.0);
-        note_info(a.status, a.blk, d_chol(wb, P + c, na, 0.0, a.Sig));
+        note_info(a.status, a.blk, d_chol(wb, P + c, na, sh, a.Sig));
         d_copy(c, wb, P, na, Rb, a.ldR);
         if (a.Cout) d_emit(a.Cout, a.n1, a.S, c, c, wb, P, na, 0, nullptr, 0);
         break;
     }
     case OP_PIPI2: {
         // Omega; R(0:c, b) += Y Sigma; R_bb = Omega Sigma (SRC/orth.c:440-455).
+        const double sh = d_pyth_shift(a, wb, P + c, na);
         d_tn(c, wb, P, na, wb, P, na, P + c, na, -1.0, 1.0);
-        note_info(a.status, a.blk, d_chol(wb, P + c, na, 0.0, a.Om));
+        note_info(a.status, a.blk, d_chol(wb, P + c, na, sh, a.Om));
         d_nn(c, wb, P, na, wb, a.Sig, kLdT, Rb, a.ldR, 1.0, 1.0);
         d_trmm(wb, a.Om, a.Sig, Rbb, a.ldR);
         if (a.Cout) d_emit(a.Cout, a.n1, a.S, c, c, wb, P, na, 0, nullptr, 0);

@@ -426,12 +426,27 @@ static void note(const ctx *c, idx b, int inf)
     }
 }
 
-/* T = chol(G) for a w x w block (upper part of G used). */
-static int chol_into(idx w, const double *G, idx ldg, double *T)
+/* T = chol(G + shift I) for a w x w block (upper part of G used). */
+static int chol_shift(idx w, const double *G, idx ldg, double shift, double *T)
 {
     for (idx j = 0; j < w; j++)
-        for (idx i = 0; i < w; i++) T[i + j * w] = (i <= j) ? G[i + j * ldg] : 0.0;
+        for (idx i = 0; i < w; i++)
+            T[i + j * w] = (i <= j) ? G[i + j * ldg] + (i == j ? shift : 0.0) : 0.0;
     return oracle_potrf(w, T, w);
+}
+
+/* Optional shift of the Pythagorean Cholesky steps (not in the reference,
+ * which factors G - S^T S as is): G - S^T S + c u tr(G) I, with u = 2^-53
+ * and G the Gram before the subtraction.  c = 0 (default) is the reference. */
+static double g_pshift = 0.0;
+void oracle_set_pythagorean_shift(double c) { g_pshift = c; }
+
+static double pyth_shift(idx w, const double *G, idx ldg)
+{
+    if (g_pshift == 0.0) return 0.0;
+    double t = 0.0;
+    for (idx i = 0; i < w; i++) t += G[i + i * ldg];
+    return g_pshift * 1.1102230246251565e-16 * t;
 }
 
 /* Second pass shared by P1s, 1s and P2s after the sync on block b:
@@ -444,8 +459,9 @@ static void second_pass(const ctx *c, idx b, idx cc, idx wb, idx wn,
     /* Omega = chol(Q^_b^T Q^_b - Y^T Y) (:134-137, :200-204, :288-291,
      * :345-349, :621-624, :676-680). */
     double *G = P + cc; /* rows cc.., cols 0..wb */
+    const double sho = pyth_shift(wb, G, na);
     tn_small(cc, wb, P, na, wb, P, na, G, na, g_plant == 2 ? 1.0 : -1.0, 1.0);
-    note(c, b, chol_into(wb, G, na, Om));
+    note(c, b, chol_shift(wb, G, na, sho, Om));
     /* Q_b = (Q^_b - Q Y) Omega^{-1} (:139-142, :206-209, :293-296, :351-354,
      * :625-628, :681-684). */
     oracle_update(c->m, wb, colp(c, cc), c->ldX, g_plant == 4 ? cc - 1 : cc,
@@ -477,9 +493,10 @@ static void run_p1s(const ctx *c)
         idx na = s + w1;
         oracle_gram(m, na, colp(c, 0), c->ldX, w1, colp(c, s), c->ldX, P, na);
         perturb(s, w1, P, na, 1);
+        const double sh = pyth_shift(w1, P + s, na);
         tn_small(s, w1, P, na, w1, P, na, P + s, na, -1.0, 1.0);
         copy_block(s, w1, P, na, Rp(c, 0, s), c->ldR); /* :117 */
-        note(c, 1, chol_into(w1, P + s, na, Sig));
+        note(c, 1, chol_shift(w1, P + s, na, sh, Sig));
         /* X2 = (X2 - Q1 S) Sigma^{-1} (:121-124). */
         oracle_update(m, w1, colp(c, s), c->ldX, s, colp(c, 0), c->ldX, P, na, Sig, w1);
     }
@@ -501,8 +518,9 @@ static void run_p1s(const ctx *c)
             /* Sigma_n = chol(X_n^T X_n - S^T S) (:160-171). */
             double *Sn = Rp(c, 0, cc + wb);
             double *H = P + wb * na + cc + wb;
+            const double shn = pyth_shift(wn, H, na);
             tn_small(cc + wb, wn, Sn, c->ldR, wn, Sn, c->ldR, H, na, -1.0, 1.0);
-            note(c, b + 1, chol_into(wn, H, na, Sig));
+            note(c, b + 1, chol_shift(wn, H, na, shn, Sig));
             /* X_n = (X_n - Q S) Sigma_n^{-1} (:173-176). */
             oracle_update(m, wn, colp(c, cc + wb), c->ldX, cc + wb, colp(c, 0),
                           c->ldX, Sn, c->ldR, Sig, wn);
@@ -566,14 +584,16 @@ static void run_pipi(const ctx *c)
         /* [Q X]^T X (:421-424); Sigma = chol(X^T X - S^T S) (:425-429). */
         oracle_gram(m, na, colp(c, 0), c->ldX, wb, colp(c, cc), c->ldX, P, na);
         perturb(cc, wb, P, na, b);
+        const double sh1 = pyth_shift(wb, P + cc, na);
         tn_small(cc, wb, P, na, wb, P, na, P + cc, na, -1.0, 1.0);
-        note(c, b, chol_into(wb, P + cc, na, Sig));
+        note(c, b, chol_shift(wb, P + cc, na, sh1, Sig));
         /* X = (X - Q S) Sigma^{-1} (:431-434). */
         oracle_update(m, wb, colp(c, cc), c->ldX, cc, colp(c, 0), c->ldX, P, na, Sig, wb);
         /* [Q X]^T X (:435-438); Omega = chol(X^T X - Y^T Y) (:440-444). */
         oracle_gram(m, na, colp(c, 0), c->ldX, wb, colp(c, cc), c->ldX, P2, na);
+        const double sh2 = pyth_shift(wb, P2 + cc, na);
         tn_small(cc, wb, P2, na, wb, P2, na, P2 + cc, na, -1.0, 1.0);
-        note(c, b, chol_into(wb, P2 + cc, na, Om));
+        note(c, b, chol_shift(wb, P2 + cc, na, sh2, Om));
         /* X = (X - Q Y) Omega^{-1} (:446-449). */
         oracle_update(m, wb, colp(c, cc), c->ldX, cc, colp(c, 0), c->ldX, P2, na, Om, wb);
         /* R(0:c, b) = S + Y Sigma; R_bb = Omega Sigma (:451-455). */

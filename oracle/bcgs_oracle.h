@@ -32,6 +32,13 @@ enum {
 /* Number of worker threads used by the row-parallel primitives (>=1).
  * Results do not depend on it: row sums use a fixed slab partition. */
 void oracle_set_threads(int nthreads);
+
+/* Optional shift of the Pythagorean Cholesky steps (P-1S, A-1S, PIPI+,
+ * P-2S second passes): chol(G - S^T S + c u tr(G) I), u = 2^-53, G the Gram
+ * before the subtraction.  c = 0 (the default) is the reference's
+ * arithmetic.  The same rule as the device library's
+ * orth_set_pythagorean_shift. */
+void oracle_set_pythagorean_shift(double c);
 int oracle_get_threads(void);
 
 /* Full orthogonalization: X (m x n, ld ldX) is overwritten by Q, R (ld ldR)

@@ -68,6 +68,8 @@ def lib():
         L.oracle_set_noise_mode.argtypes = [ctypes.c_int]
         L.oracle_plant.restype = None
         L.oracle_plant.argtypes = [ctypes.c_int]
+        L.oracle_set_pythagorean_shift.restype = None
+        L.oracle_set_pythagorean_shift.argtypes = [ctypes.c_double]
         L.oracle_set_threads.restype = None
         L.oracle_set_threads.argtypes = [ctypes.c_int]
         L.oracle_get_threads.restype = ctypes.c_int
@@ -93,6 +95,11 @@ def set_first_pass_noise(eta, column_scaled=False):
     a dot product computed in reduced precision."""
     lib().oracle_set_noise_mode(1 if column_scaled else 0)
     lib().oracle_set_first_pass_noise(float(eta))
+
+
+def set_pythagorean_shift(c):
+    """Shift the Pythagorean Cholesky steps by c u tr(G) (0 removes it)."""
+    lib().oracle_set_pythagorean_shift(float(c))
 
 
 def set_threads(n):

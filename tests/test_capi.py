@@ -108,3 +108,10 @@ def test_batched_rejects_invalid_without_device():
                                     arr(vp, [16]), arr(i64, [1]), arr(vp, [16]), arr(i64, [1]),
                                     None, 0, None, None)
     assert rc == 2
+
+
+def test_pythagorean_shift_argument_checked_without_device():
+    L = B.lib()
+    for bad in (-1.0, float("nan"), float("inf")):
+        assert L.orth_set_pythagorean_shift(bad) == 2
+    assert L.orth_set_pythagorean_shift(0.0) == 0

@@ -298,3 +298,49 @@ def test_padded_leading_dimension(variant, m, ld):
     assert info == (0, 0)
     assert np.abs(R - Ro).max() <= 1e-11 * np.abs(Ro).max()
     assert np.abs(Qp[:m] - Qo).max() <= 1e-10
+
+
+@pytest.mark.parametrize("variant", ["bcgsi2P1s", "bcgspipi2", "bcgsi21s", "bcgsi2P2s"])
+def test_pythagorean_shift(variant):
+    """The optional Pythagorean shift: GPU against the oracle with the same
+    shift, and R's diagonal against its closed form on an input with known
+    first-pass Grams."""
+    m, n, s, c = 20000, 64, 16, 1.0e7
+    X, d = I.layered_matrix(m, n, s)
+    say("case pythagorean shift", variant)
+    B.set_pythagorean_shift(c)
+    O.set_pythagorean_shift(c)
+    try:
+        Qg, Rg, st = B.bcgs(variant, X, s)
+        Qo, Ro, info_o = O.bcgs(variant, X, s)
+    finally:
+        B.set_pythagorean_shift(0.0)
+        O.set_pythagorean_shift(0.0)
+    assert st["hip_error"] == 0 and (st["info_block"], st["info_pivot"]) == info_o == (0, 0)
+    assert np.abs(Rg - Ro).max() <= 1e-11 * np.abs(Ro).max()
+    assert np.abs(Qg - Qo).max() <= 1e-10
+    if variant != "bcgsi2P2s":
+        want = I.shifted_diag(variant, X, d, s, c)
+        assert np.abs(np.abs(np.diag(Rg)) - want).max() <= 1e-12 * want.max()
+
+
+def test_pythagorean_shift_glued():
+    """Glued blocks at delta = 1e-13: P-1S breaks down without the shift and
+    completes with it, as in the oracle."""
+    m, n, s = 16384, 128, 16
+    X = I.glued_matrix(m, n, s, 1e-13, seed=36)
+    say("case pythagorean shift glued")
+    _, _, st0 = B.bcgs("bcgsi2P1s", X, s)
+    assert st0["info_block"] > 0
+    c = 11.0 * (m * s + s * (s + 1))
+    B.set_pythagorean_shift(c)
+    O.set_pythagorean_shift(c)
+    try:
+        Qg, Rg, st = B.bcgs("bcgsi2P1s", X, s)
+        _, _, info_o = O.bcgs("bcgsi2P1s", X, s)
+    finally:
+        B.set_pythagorean_shift(0.0)
+        O.set_pythagorean_shift(0.0)
+    assert st["info_block"] == 0 and info_o == (0, 0)
+    assert np.all(np.isfinite(Qg)) and np.all(np.isfinite(Rg))
+

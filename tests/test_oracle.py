@@ -248,6 +248,43 @@ def test_glued_blocks():
     assert info[0] == 2
 
 
+@pytest.mark.parametrize("variant", ["bcgsi2P1s", "bcgspipi2", "bcgsi21s"])
+def test_pythagorean_shift_closed_form(variant):
+    """The optional shift c u tr(G) of the Pythagorean Cholesky steps, on an
+    input whose first-pass Gram X^T X - S^T S is diag(d)^2 exactly: R's
+    diagonal follows the closed form (tr(G) includes the projected part, so
+    a shift taken after the subtraction would fail), and c = 0 gives |d|."""
+    m, n, s, c = 3000, 40, 8, 1.0e7
+    X, d = I.layered_matrix(m, n, s)
+    O.set_pythagorean_shift(c)
+    try:
+        Q, R, info = O.bcgs(variant, X, s)
+    finally:
+        O.set_pythagorean_shift(0.0)
+    assert info == (0, 0)
+    want = I.shifted_diag(variant, X, d, s, c)
+    got = np.abs(np.diag(R))
+    assert np.abs(got - want).max() <= 1e-12 * want.max()
+    # The shift is visible (relative effect ~1e-9 >> rounding).
+    assert np.abs(got[s:] - d[s:]).max() > 1e-10
+    Q0, R0, _ = O.bcgs(variant, X, s)
+    assert np.abs(np.abs(np.diag(R0)) - d).max() <= 1e-13 * d.max()
+
+
+def test_pythagorean_shift_avoids_breakdown():
+    """Glued blocks at delta = 1e-13 break P-1S's Pythagorean Cholesky; with
+    the shift 11 (m s + s (s + 1)) u tr(G) every factorization is defined."""
+    m, n, s = 4000, 64, 8
+    X = I.glued_matrix(m, n, s, 1e-13)
+    O.set_pythagorean_shift(11.0 * (m * s + s * (s + 1)))
+    try:
+        Q, R, info = O.bcgs("bcgsi2P1s", X, s)
+    finally:
+        O.set_pythagorean_shift(0.0)
+    assert info == (0, 0)
+    assert np.all(np.isfinite(Q)) and np.all(np.isfinite(R))
+
+
 def test_accuracy_matches_numpy():
     """Harness values on a factorization perturbed well above rounding level,
     so both LOO and the residual are exactly representable targets."""
