@@ -161,9 +161,36 @@ def run_one(torch, B, variant, X0, m, n, s, reps=3):
     return sorted(times)[len(times) // 2], status, loo, res / xn
 
 
+def run_mixed(torch, B, prec, X0, X32, m, n, s, reps=2):
+    """Time `reps` mixed-form factorizations of the FP32 input X32; returns
+    (ms, status, loo, rel_residual) of the last one, measured against the
+    FP64 input X0."""
+    Q = torch.empty_like(X0)
+    R = torch.zeros(n * n, dtype=torch.float64, device="cuda")
+    st = torch.cuda.current_stream()
+    times = []
+    for k in range(reps + 1):
+        R.zero_()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        B.bcgs_mixed_device("bcgsi2P1s", prec, m, n, s, X32.data_ptr(), m, Q.data_ptr(), m,
+                            R.data_ptr(), n, st.cuda_stream)
+        e1.record(st)
+        torch.cuda.synchronize()
+        if k:
+            times.append(e0.elapsed_time(e1))
+    status = B.status()
+    loo, res, xn = B.accuracy_device(m, n, X0.data_ptr(), m, Q.data_ptr(), m, R.data_ptr(), n,
+                                     st.cuda_stream)
+    del Q, R
+    return sorted(times)[len(times) // 2], status, loo, res / xn
+
+
 def suite(args):
     """The measurement configurations beside the headline line: every variant
-    at m=2^24, n=256, s=16 for kappa in {1e2, 1e6, 1e10}; the reference's CPU
+    at m=2^24, n=256, s=16 for kappa in {1e2, 1e6, 1e10}; the mixed form at
+    the same shape for kappa in {1e2, 1e6, 1e10, 1e14} and each first-pass
+    precision, beside FP64 BCGSI+P-1S on the same input; the reference's CPU
     shape (m=1e5, n=200, s=10, kappa=1e8) on the GPU beside the CPU oracle on
     the same input; and glued near-rank-deficient blocks at m=2^22, n=512,
     s=32 for delta in {1e-6, 1e-10, 1e-13}.  One JSON object per case on
@@ -200,17 +227,32 @@ def suite(args):
               "cpu_rel_residual": inputs.rel_residual(Xh, Qo, Ro), "cpu_info_block": info[0]})
     del X0
 
-    # North-star shape, every variant, three conditionings.
+    # North-star shape, every variant, three conditionings; the mixed form
+    # (BCGSI+P-1S on the FP32-rounded input, four first-pass precisions)
+    # beside FP64 BCGSI+P-1S on the same input, at four conditionings.
     m, n, s = 1 << 24, 256, 16
-    for kappa in (1e2, 1e6, 1e10):
+    for kappa in (1e2, 1e6, 1e10, 1e14):
         X0 = make_input(torch, m, n, kappa, seed=1)
-        for v in variants:
+        fp64_p1s = None
+        for v in (variants if kappa <= 1e10 else ["bcgsi2P1s"]):
             ms, st, loo, res = run_one(torch, B, v, X0, m, n, s)
-            emit({"config": "fp64", "variant": v, "m": m, "n": n, "s": s, "kappa": kappa,
-                  "ms": round(ms, 3), "GBs_alg": round(b_alg(m, n, s) / (ms / 1e3) / 1e9, 1),
+            if v == "bcgsi2P1s":
+                fp64_p1s = (ms, loo)
+            if kappa <= 1e10:
+                emit({"config": "fp64", "variant": v, "m": m, "n": n, "s": s, "kappa": kappa,
+                      "ms": round(ms, 3), "GBs_alg": round(b_alg(m, n, s) / (ms / 1e3) / 1e9, 1),
+                      "loo": loo, "rel_residual": res, "info_block": st["info_block"],
+                      "passes": st["passes"], "syncs": st["syncs"]})
+        X32 = X0.float()
+        for prec in ("fp64", "bf16", "fp16", "bf16x3"):
+            ms, st, loo, res = run_mixed(torch, B, prec, X0, X32, m, n, s)
+            emit({"config": "mixed", "precision": prec, "m": m, "n": n, "s": s,
+                  "kappa": kappa, "ms": round(ms, 3),
+                  "GBs_alg": round(b_alg_mixed(m, n, s) / (ms / 1e3) / 1e9, 1),
                   "loo": loo, "rel_residual": res, "info_block": st["info_block"],
-                  "passes": st["passes"], "syncs": st["syncs"]})
-        del X0
+                  "fp64_ms": round(fp64_p1s[0], 3), "fp64_loo": fp64_p1s[1],
+                  "speedup_vs_fp64": round(fp64_p1s[0] / ms, 3)})
+        del X0, X32
         torch.cuda.empty_cache()
 
     # Glued stress.
