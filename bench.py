@@ -369,7 +369,8 @@ def batched(args):
     # Mixed form on the s = 16 instances (its supported shapes).
     idx = [i for i in range(len(shapes)) if shapes[i][2] == 16]
     X32 = {i: X0[i].float() for i in idx}
-    for prec in ("fp64", "bf16", "bf16x3"):
+    # Only FP64 products: the batched form rejects the reduced precisions.
+    for prec in ("fp64",):
         d, _ = measure("bcgsi2P1s-" + prec, idx, lambda tune: B.bcgs_batched_device(
             "bcgsi2P1s", [ms_[i] for i in idx], [ns_[i] for i in idx], [16] * len(idx),
             [X[i].data_ptr() for i in idx], [ms_[i] for i in idx],

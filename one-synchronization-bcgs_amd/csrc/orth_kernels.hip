@@ -261,6 +261,9 @@ __device__ void d_tn(int k, int ma, const double *A, int64_t lda, int nb, const 
     for (int e = threadIdx.x; e < ma * nb; e += blockDim.x) {
         const int i = e % ma, j = e / ma;
         double s = 0.0;
+        // Unrolled so that the loads of several terms are in flight; the
+        // summation order stays sequential.
+#pragma unroll 8
         for (int l = 0; l < k; l++) s = fma(A[l + i * lda], B[l + j * ldb], s);
         C[i + j * ldc] = (beta == 0.0 ? 0.0 : beta * C[i + j * ldc]) + alpha * s;
     }
@@ -274,6 +277,7 @@ __device__ void d_nn(int ma, int k, const double *A, int64_t lda, int nb, const 
     for (int e = threadIdx.x; e < ma * nb; e += blockDim.x) {
         const int i = e % ma, j = e / ma;
         double s = 0.0;
+#pragma unroll 8
         for (int l = 0; l < k; l++) s = fma(A[i + l * lda], B[l + j * ldb], s);
         C[i + j * ldc] = (beta == 0.0 ? 0.0 : beta * C[i + j * ldc]) + alpha * s;
     }
@@ -449,8 +453,18 @@ __device__ double d_pyth_shift(const SmallArgs &a, int w, const double *G, int64
 
 __global__ __launch_bounds__(256) void k_small(SmallArgs a)
 {
+    // P is read many times by the dot products below: stage it in LDS (the
+    // steps then see LDS latency instead of HBM/L2 latency) and write it
+    // back at the end for the kernels that read it afterwards.
+    extern __shared__ double smP[];
     double *P = a.P;
     const int c = a.c, wb = a.wb, wn = a.wn, na = a.na;
+    const int pn = a.stage ? na * small_pcols(a.op, wb, wn) : 0;
+    if (pn > 0) {
+        for (int e = threadIdx.x; e < pn; e += blockDim.x) smP[e] = a.P[e];
+        __syncthreads();
+        P = smP;
+    }
     double *Rb = a.R + (int64_t)c * a.ldR;         // R(:, block b)
     double *Rbb = Rb + c;                          // R(b, b)
     switch (a.op) {
@@ -579,11 +593,30 @@ __global__ __launch_bounds__(256) void k_small(SmallArgs a)
     }
     default: break;
     }
+    if (pn > 0) {
+        __syncthreads();
+        for (int e = threadIdx.x; e < pn; e += blockDim.x) a.P[e] = smP[e];
+    }
 }
 
-hipError_t launch_small(const SmallArgs &a, hipStream_t st)
+// Largest P staged in LDS by k_small (beside its static shared memory).
+constexpr size_t kSmallStageBytes = 96 * 1024;
+
+hipError_t launch_small(const SmallArgs &a0, hipStream_t st)
 {
-    k_small<<<dim3(1), dim3(256), 0, st>>>(a);
+    static bool attr_set = false;
+    if (!attr_set) {
+        hipError_t e = hipFuncSetAttribute((const void *)k_small,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)kSmallStageBytes);
+        if (e != hipSuccess) return e;
+        attr_set = true;
+    }
+    SmallArgs a = a0;
+    const size_t bytes =
+        a.P ? (size_t)a.na * small_pcols(a.op, a.wb, a.wn) * sizeof(double) : 0;
+    a.stage = (bytes > 0 && bytes <= kSmallStageBytes && !getenv("ORTH_SMALL_GLOBAL")) ? 1 : 0;
+    k_small<<<dim3(1), dim3(256), a.stage ? bytes : 0, st>>>(a);
     return hipGetLastError();
 }
 
