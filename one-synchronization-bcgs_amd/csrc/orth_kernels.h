@@ -79,13 +79,7 @@ struct SmallArgs {
     // Pythagorean Cholesky steps factor G - S^T S + pshift * u * tr(G) * I
     // (u = 2^-53, G the Gram before the subtraction); 0 = no shift.
     double pshift;
-    // 1: P (na x small_pcols(op, wb, wn)) is staged in LDS for the step and
-    // written back at its end (set by launch_small when it fits).
-    int stage;
 };
-
-// Columns of P a small step reads or writes.
-__host__ __device__ inline int small_pcols(int op, int wb, int wn) { return op == OP_CLOSE_OPEN ? wb + wn : wb; }
 
 hipError_t launch_update(int S, double *X, int64_t ldX, int64_t m,
                          const Upd *u, int nupd, hipStream_t st);

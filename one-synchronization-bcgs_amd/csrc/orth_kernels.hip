@@ -453,18 +453,8 @@ __device__ double d_pyth_shift(const SmallArgs &a, int w, const double *G, int64
 
 __global__ __launch_bounds__(256) void k_small(SmallArgs a)
 {
-    // P is read many times by the dot products below: stage it in LDS (the
-    // steps then see LDS latency instead of HBM/L2 latency) and write it
-    // back at the end for the kernels that read it afterwards.
-    extern __shared__ double smP[];
     double *P = a.P;
     const int c = a.c, wb = a.wb, wn = a.wn, na = a.na;
-    const int pn = a.stage ? na * small_pcols(a.op, wb, wn) : 0;
-    if (pn > 0) {
-        for (int e = threadIdx.x; e < pn; e += blockDim.x) smP[e] = a.P[e];
-        __syncthreads();
-        P = smP;
-    }
     double *Rb = a.R + (int64_t)c * a.ldR;         // R(:, block b)
     double *Rbb = Rb + c;                          // R(b, b)
     switch (a.op) {
@@ -593,30 +583,11 @@ __global__ __launch_bounds__(256) void k_small(SmallArgs a)
     }
     default: break;
     }
-    if (pn > 0) {
-        __syncthreads();
-        for (int e = threadIdx.x; e < pn; e += blockDim.x) a.P[e] = smP[e];
-    }
 }
 
-// Largest P staged in LDS by k_small (beside its static shared memory).
-constexpr size_t kSmallStageBytes = 96 * 1024;
-
-hipError_t launch_small(const SmallArgs &a0, hipStream_t st)
+hipError_t launch_small(const SmallArgs &a, hipStream_t st)
 {
-    static bool attr_set = false;
-    if (!attr_set) {
-        hipError_t e = hipFuncSetAttribute((const void *)k_small,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           (int)kSmallStageBytes);
-        if (e != hipSuccess) return e;
-        attr_set = true;
-    }
-    SmallArgs a = a0;
-    const size_t bytes =
-        a.P ? (size_t)a.na * small_pcols(a.op, a.wb, a.wn) * sizeof(double) : 0;
-    a.stage = (bytes > 0 && bytes <= kSmallStageBytes && !getenv("ORTH_SMALL_GLOBAL")) ? 1 : 0;
-    k_small<<<dim3(1), dim3(256), a.stage ? bytes : 0, st>>>(a);
+    k_small<<<dim3(1), dim3(256), 0, st>>>(a);
     return hipGetLastError();
 }
 
