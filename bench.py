@@ -161,7 +161,7 @@ def run_one(torch, B, variant, X0, m, n, s, reps=3):
     return sorted(times)[len(times) // 2], status, loo, res / xn
 
 
-def run_mixed(torch, B, prec, X0, X32, m, n, s, reps=2):
+def run_mixed(torch, B, prec, X0, X32, m, n, s, reps=2, variant="bcgsi2P1s"):
     """Time `reps` mixed-form factorizations of the FP32 input X32; returns
     (ms, status, loo, rel_residual) of the last one, measured against the
     FP64 input X0."""
@@ -173,7 +173,7 @@ def run_mixed(torch, B, prec, X0, X32, m, n, s, reps=2):
         R.zero_()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record(st)
-        B.bcgs_mixed_device("bcgsi2P1s", prec, m, n, s, X32.data_ptr(), m, Q.data_ptr(), m,
+        B.bcgs_mixed_device(variant, prec, m, n, s, X32.data_ptr(), m, Q.data_ptr(), m,
                             R.data_ptr(), n, st.cuda_stream)
         e1.record(st)
         torch.cuda.synchronize()
@@ -276,6 +276,15 @@ def suite(args):
                   "rel_residual": res, "info_block": st["info_block"],
                   "info_pivot": st["info_pivot"]})
         B.set_pythagorean_shift(0.0)
+        # Mixed form with FP64 products on the FP32-rounded input (widened
+        # into Q, then the FP64 form: n = 512 is outside the fused mixed pass).
+        X32 = X0.float()
+        for v in ("bcgsi2P1s", "bcgsi2"):
+            ms, st, loo, res = run_mixed(torch, B, "fp64", X0, X32, m, n, s, reps=1, variant=v)
+            emit({"config": "glued-mixed", "variant": v, "precision": "fp64", "m": m, "n": n,
+                  "s": s, "delta": delta, "ms": round(ms, 3), "loo": loo, "rel_residual": res,
+                  "info_block": st["info_block"], "info_pivot": st["info_pivot"]})
+        del X32
         del X0
         torch.cuda.empty_cache()
     return out

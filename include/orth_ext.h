@@ -64,8 +64,11 @@ int orth_bcgs_device(int variant, int64_t m, int64_t n, int64_t s, double *dX,
  * tile, ORTH_PREC_BF16X3 (FP32 values split into two bf16 terms, three
  * products), or ORTH_PREC_FP64.  The reorthogonalization (second-pass
  * update and projection), the Pythagorean subtraction, the Cholesky factors
- * and the triangular solves are FP64 in every case.  Supports variant
- * ORTH_P1S with s = 16 and n <= 256.  Returns 0, or 2 for an unsupported
+ * and the triangular solves are FP64 in every case.  The reduced
+ * precisions support variant ORTH_P1S with s = 16 and n <= 256 (the fused
+ * mixed pass).  ORTH_PREC_FP64 is accepted for every variant and shape of
+ * orth_bcgs_device: outside those limits the input is widened exactly into
+ * Q and the FP64 form runs in place.  Returns 0, or 2 for an unsupported
  * shape / precision or a HIP error; breakdown is reported by
  * orth_last_status() as for the FP64 form. */
 int orth_bcgs_mixed_device(int variant, int precision, int64_t m, int64_t n,

@@ -1367,7 +1367,12 @@ int orth_bcgs_mixed_device(int variant, int precision, int64_t m, int64_t n, int
                            int64_t ldR, void *stream)
 {
     if (!dX32 || !dQ || !dR || !shape_ok(m, n, s, ldQ, ldR) || ldX32 < m) return 2;
-    if (variant != ORTH_P1S || s != 16 || n > 256) return 2;
+    if (variant < ORTH_P1S || variant > ORTH_P2S) return 2;
+    // The fused mixed pass covers BCGSI+P-1S with s = 16 and n <= 256; FP64
+    // products on FP32 storage are also accepted for every other variant
+    // and shape (the input is widened into Q, then the FP64 form runs).
+    const bool fused_shape = variant == ORTH_P1S && s == 16 && n <= 256;
+    if (!fused_shape && precision != ORTH_PREC_FP64) return 2;
     int mx = 0;
     switch (precision) {
     case ORTH_PREC_FP64: mx = kMxFP64; break;
@@ -1381,6 +1386,11 @@ int orth_bcgs_mixed_device(int variant, int precision, int64_t m, int64_t n, int
     DevCtx &c = g_ctx[dev];
     std::lock_guard<std::mutex> lk(c.mu);
     if (!ctx_init(c)) return 2;
+    if (!fused_shape) {
+        if (!ok(launch_widen(dX32, ldX32, dQ, ldQ, m, n, (hipStream_t)stream))) return 2;
+        return run_device(c, variant, m, n, s, dQ, ldQ, dR, ldR, hipMemcpyDeviceToDevice,
+                          (hipStream_t)stream);
+    }
     return run_device(c, variant, m, n, s, dQ, ldQ, dR, ldR, hipMemcpyDeviceToDevice,
                       (hipStream_t)stream, mx, dX32, ldX32);
 }

@@ -175,6 +175,10 @@ hipError_t launch_pass16m(const PassArgs &a, int mx, int nwg, hipStream_t st);
 int pass16bm_plan(const PassArgs &a);
 hipError_t launch_pass16bm(const PassArgs &a, int mx, int nwg, hipStream_t st);
 
+// Q = X32 widened exactly to FP64 (m x n, column-major).
+hipError_t launch_widen(const float *x, int64_t ldx, double *q, int64_t ldq, int64_t m, int64_t n,
+                        hipStream_t st);
+
 // Zero n entries with a kernel (no memset nodes, so the sequence can be
 // captured in a graph).
 hipError_t launch_fill(double *p, size_t n, hipStream_t st);

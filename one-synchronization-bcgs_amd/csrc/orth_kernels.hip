@@ -13,6 +13,7 @@
 #include <math.h>
 #include <stdlib.h>
 
+#include <algorithm>
 #include <type_traits>
 
 #include "orth_device.h"
@@ -1164,6 +1165,28 @@ hipError_t launch_pass(int S, const PassArgs &a, int nwg, hipStream_t st)
     case 32: return launch_pass_t<32>(a, nwg, st);
     default: return hipErrorInvalidValue;
     }
+}
+
+// Q (m x n, ld ldq) = X32 widened exactly to FP64.
+__global__ __launch_bounds__(256) void k_widen(const float *__restrict__ x, int64_t ldx,
+                                                double *__restrict__ q, int64_t ldq, int64_t m,
+                                                int64_t n)
+{
+    const int64_t total = m * n;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = e % m, j = e / m;
+        q[i + j * ldq] = (double)x[i + j * ldx];
+    }
+}
+
+hipError_t launch_widen(const float *x, int64_t ldx, double *q, int64_t ldq, int64_t m, int64_t n,
+                        hipStream_t st)
+{
+    const int64_t total = m * n;
+    const int nb = (int)std::min<int64_t>((total + 255) / 256, 8192);
+    k_widen<<<dim3(std::max(nb, 1)), dim3(256), 0, st>>>(x, ldx, q, ldq, m, n);
+    return hipGetLastError();
 }
 
 template <typename T>

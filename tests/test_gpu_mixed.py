@@ -27,14 +27,14 @@ def say(*a):
     print(*a, flush=True)
 
 
-def mixed(X, s, precision):
+def mixed(X, s, precision, variant="bcgsi2P1s"):
     """Run the mixed form on X rounded to FP32; returns (Q, R, status)."""
     m, n = X.shape
     X32 = np.asfortranarray(X.astype(np.float32))
     x32 = torch.from_numpy(np.ascontiguousarray(X32.T)).cuda()
     q = torch.zeros((n, m), dtype=torch.float64, device="cuda")
     r = torch.zeros(n * n, dtype=torch.float64, device="cuda")
-    B.bcgs_mixed_device("bcgsi2P1s", precision, m, n, s, x32.data_ptr(), m, q.data_ptr(), m,
+    B.bcgs_mixed_device(variant, precision, m, n, s, x32.data_ptr(), m, q.data_ptr(), m,
                         r.data_ptr(), n)
     B.sync()
     torch.cuda.synchronize()
@@ -67,6 +67,22 @@ def test_fp64_products_match_oracle(shape, kernel, monkeypatch):
     assert np.abs(R - Ro).max() <= 1e-11 * np.abs(Ro).max()
     assert np.abs(Q - Qo).max() <= 1e-10
     assert np.all(np.tril(R, -1) == 0)
+
+
+@pytest.mark.parametrize("case", [("bcgsi2", 5000, 80, 16), ("bcgspipi2", 6000, 64, 8),
+                                  ("bcgsi2P1s", 20000, 512, 32), ("bcgsi2P2s", 4000, 40, 10)])
+def test_fp64_products_other_variants_and_shapes(case):
+    """FP64 products on FP32 storage outside the fused mixed pass (other
+    variants, s != 16, n > 256): the input is widened into Q and the FP64
+    form runs; Q and R match the oracle on the rounded input."""
+    variant, m, n, s = case
+    X = I.svd_matrix(m, n, 1e4, seed=43)
+    say("case mixed fp64 widened", case)
+    Q, R, st, Xr = mixed(X, s, "fp64", variant)
+    Qo, Ro, info = O.bcgs(variant, Xr, s)
+    assert st["hip_error"] == 0 and (st["info_block"], st["info_pivot"]) == info == (0, 0)
+    assert np.abs(R - Ro).max() <= 1e-11 * np.abs(Ro).max()
+    assert np.abs(Q - Qo).max() <= 1e-10
 
 
 LEVEL = {"bf16": 2.0 ** -9, "fp16": 2.0 ** -12, "bf16x3": 2.0 ** -18}
