@@ -558,8 +558,10 @@ class Run {
         // The mixed pass with reduced-precision products runs one 512-thread
         // workgroup per CU (its operand conversions need two waves' worth of
         // registers); with FP64 products it runs two, like the FP64 pass.
-        int nwg = std::max(
-            1, (int)(std::max(1, c_.ncu) * (ts16 > 0 || mx_ == kMxFP64 ? 2 : 1) * wg_scale_));
+        // The 256-thread passes keep as many workgroups per CU as their
+        // registers and LDS allow.
+        const int per_cu = (ts16 > 0 || mx_ == kMxFP64) ? 2 : mx_ ? 1 : pass_wg_per_cu(S_, pa);
+        int nwg = std::max(1, (int)(std::max(1, c_.ncu) * per_cu * wg_scale_));
         int64_t rows = (m_ + nwg - 1) / nwg;
         rows = (rows + unit - 1) / unit * unit;
         pa.rows_per = rows;

@@ -618,8 +618,16 @@ size_t pass_lds_bytes(const PassArgs &a)
     return ((size_t)a.wl * kPLd + (size_t)k1p * a.n1 + 2 * kMaxS * kMaxS) * sizeof(double);
 }
 
-template <int S>
-__global__ __launch_bounds__(256) void k_pass(PassArgs a)
+// Waves per SIMD (= 256-thread workgroups per CU) the generic pass is
+// compiled for: the wide instance uses up to 512 registers per lane (one
+// workgroup per CU); the narrow one fits three (S <= 8) or two (S = 16).
+constexpr int pass_waves(int S, int MT) { return MT > 4 || S > 16 ? 1 : S <= 8 ? 3 : 2; }
+
+// MT: projection tiles per wave, PF: prefetch registers per thread.  The
+// narrow instance (fewer of both) fits several workgroups per CU.
+template <int S, int MT = kPassMaxTiles, int PF = kPF>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
+    pass_waves(S, MT), pass_waves(S, MT)))) void k_pass(PassArgs a)
 {
     extern __shared__ double sm[];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -645,19 +653,19 @@ __global__ __launch_bounds__(256) void k_pass(PassArgs a)
     const int Ma = (a.na + 15) >> 4, Nb = (a.nb + 15) >> 4;
     const int ntiles = a.na > 0 ? Ma * Nb : 0;
 
-    dbl4 acc[kPassMaxTiles];
+    dbl4 acc[MT];
 #pragma unroll
-    for (int t = 0; t < kPassMaxTiles; t++) acc[t] = dbl4{0.0, 0.0, 0.0, 0.0};
+    for (int t = 0; t < MT; t++) acc[t] = dbl4{0.0, 0.0, 0.0, 0.0};
 
     const int64_t rb = (int64_t)blockIdx.x * a.rows_per;
     const int64_t re = min(a.m, rb + a.rows_per);
     const int nload = a.ncols * kPT;
-    double pf[kPF];
+    double pf[PF];
     // Element e of a tile is column e / 32, row e % 32: each half-wave reads
     // 32 consecutive rows (256 bytes) of one column.
     auto load = [&](int64_t r0) {
 #pragma unroll
-        for (int i = 0; i < kPF; i++) {
+        for (int i = 0; i < PF; i++) {
             const int e = tid + 256 * i;
             if (e < nload) {
                 const int64_t row = r0 + (e & (kPT - 1));
@@ -671,7 +679,7 @@ __global__ __launch_bounds__(256) void k_pass(PassArgs a)
     for (int64_t r0 = rb; r0 < re; r0 += kPT) {
         __syncthreads();
 #pragma unroll
-        for (int i = 0; i < kPF; i++) {
+        for (int i = 0; i < PF; i++) {
             const int e = tid + 256 * i;
             if (e < nload) tile[(e / kPT) * kPLd + (e & (kPT - 1))] = pf[i];
         }
@@ -770,7 +778,7 @@ __global__ __launch_bounds__(256) void k_pass(PassArgs a)
             for (int ks = 0; ks < kPT / 4; ks++) {
                 const int rr = 4 * ks + lk;
 #pragma unroll
-                for (int t = 0; t < kPassMaxTiles; t++) {
+                for (int t = 0; t < MT; t++) {
                     const int tl = wave + 4 * t;
                     if (tl < ntiles) {
                         const int ta = tl / Nb, tb = tl - ta * Nb;
@@ -786,7 +794,7 @@ __global__ __launch_bounds__(256) void k_pass(PassArgs a)
     if (ntiles > 0) {
         double *out = a.part + (int64_t)blockIdx.x * a.na * a.nb;
 #pragma unroll
-        for (int t = 0; t < kPassMaxTiles; t++) {
+        for (int t = 0; t < MT; t++) {
             const int tl = wave + 4 * t;
             if (tl < ntiles) {
                 const int ta = tl / Nb, tb = tl - ta * Nb;
@@ -1134,21 +1142,51 @@ hipError_t launch_pass16(const PassArgs &a, size_t lds, int nwg, hipStream_t st)
 #undef ORTH_P16
 }
 
-template <int S>
-hipError_t launch_pass_t(const PassArgs &a, int nwg, hipStream_t st)
+// The narrow generic pass: at most kNarrowCols columns and 4 projection
+// tiles per wave.
+constexpr int kNarrowCols = 160;
+constexpr int kNarrowPF = kNarrowCols * kPT / 256;
+
+bool pass_narrow(const PassArgs &a)
+{
+    const int Ma = (a.na + 15) / 16, Nb = (a.nb + 15) / 16;
+    return a.ncols <= kNarrowCols && (a.na == 0 || Ma * Nb <= 4 * 4);
+}
+
+template <int S, int MT, int PF>
+hipError_t launch_pass_v(const PassArgs &a, size_t lds, int nwg, hipStream_t st)
 {
     static bool attr_set = false;
-    const size_t lds = pass_lds_bytes(a);
-    if (lds == 0) return hipErrorInvalidValue;
     if (!attr_set) {
-        hipError_t e = hipFuncSetAttribute((const void *)k_pass<S>,
+        hipError_t e = hipFuncSetAttribute((const void *)k_pass<S, MT, PF>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize,
                                            (int)(160 * 1024));
         if (e != hipSuccess) return e;
         attr_set = true;
     }
-    k_pass<S><<<dim3(nwg), dim3(256), lds, st>>>(a);
+    k_pass<S, MT, PF><<<dim3(nwg), dim3(256), lds, st>>>(a);
     return hipGetLastError();
+}
+
+template <int S>
+hipError_t launch_pass_t(const PassArgs &a, int nwg, hipStream_t st)
+{
+    const size_t lds = pass_lds_bytes(a);
+    if (lds == 0) return hipErrorInvalidValue;
+    if (pass_narrow(a)) return launch_pass_v<S, 4, kNarrowPF>(a, lds, nwg, st);
+    return launch_pass_v<S, kPassMaxTiles, kPF>(a, lds, nwg, st);
+}
+
+int pass_wg_per_cu(int S, const PassArgs &a)
+{
+    const size_t lds = pass_lds_bytes(a);
+    if (lds == 0) return 1;
+    // Register-limited residency (k_pass16 and the wide generic pass: one
+    // workgroup per CU).
+    const bool p16 = S == 16 && pass16_mt(a) > 0 && !getenv("ORTH_GENERIC_PASS");
+    const int by_regs = p16 ? 1 : pass_narrow(a) ? pass_waves(S, 4) : pass_waves(S, kPassMaxTiles);
+    const int by_lds = (int)std::max<size_t>(1, (160 * 1024) / lds);
+    return std::max(1, std::min(by_regs, by_lds));
 }
 
 hipError_t launch_pass(int S, const PassArgs &a, int nwg, hipStream_t st)
