@@ -375,18 +375,7 @@ def batched(args):
         d["oracle_sample"] = [{"gpu_loo": a, "cpu_loo": b} for a, b in ol]
         print(json.dumps(finite(d)), flush=True)
 
-    # Mixed form on the s = 16 instances (its supported shapes).
-    idx = [i for i in range(len(shapes)) if shapes[i][2] == 16]
-    X32 = {i: X0[i].float() for i in idx}
-    # Only FP64 products: the batched form rejects the reduced precisions.
-    for prec in ("fp64",):
-        d, _ = measure("bcgsi2P1s-" + prec, idx, lambda tune: B.bcgs_batched_device(
-            "bcgsi2P1s", [ms_[i] for i in idx], [ns_[i] for i in idx], [16] * len(idx),
-            [X[i].data_ptr() for i in idx], [ms_[i] for i in idx],
-            [R[i].data_ptr() for i in idx], [ns_[i] for i in idx], tune=tune,
-            precision=prec, x32_ptrs=[X32[i].data_ptr() for i in idx],
-            ldx32s=[ms_[i] for i in idx]))
-        print(json.dumps(finite(d)), flush=True)
+    # The batched mixed form is switched off (the entry point returns 2).
 
 
 def main():

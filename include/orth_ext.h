@@ -100,9 +100,8 @@ int orth_bcgs_batched_device(int variant, int count, const int64_t *m,
                              int tune, int *config, void *stream);
 
 /* The batched form of orth_bcgs_mixed_device: FP32 inputs dX32[i], FP64
- * outputs dQ[i] and dR[i]; every instance must satisfy that function's
- * limits (ORTH_P1S, s = 16, n <= 256).  Only ORTH_PREC_FP64 is accepted in
- * this version; the reduced precisions return 2. */
+ * outputs dQ[i] and dR[i].  Switched off in this version: every call
+ * returns 2 (use orth_bcgs_mixed_device per instance). */
 int orth_bcgs_batched_mixed_device(int variant, int precision, int count,
                                    const int64_t *m, const int64_t *n,
                                    const int64_t *s, const float *const *dX32,

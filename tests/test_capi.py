@@ -117,14 +117,14 @@ def test_pythagorean_shift_argument_checked_without_device():
     assert L.orth_set_pythagorean_shift(0.0) == 0
 
 
-def test_batched_mixed_reduced_precision_rejected_without_device():
-    """The batched mixed form accepts only FP64 products (returns 2 for
-    bf16 / fp16 / bf16x3 before touching the device)."""
+def test_batched_mixed_rejected_without_device():
+    """The batched mixed form is switched off: every precision returns 2
+    before touching the device."""
     i64 = ctypes.c_int64
     vp = ctypes.c_void_p
     L = B.lib()
     arr = lambda t, v: (t * len(v))(*v)
-    for prec in ("bf16", "fp16", "bf16x3"):
+    for prec in ("fp64", "bf16", "fp16", "bf16x3"):
         rc = L.orth_bcgs_batched_mixed_device(
             0, B.PRECISIONS[prec], 1, arr(i64, [4096]), arr(i64, [32]), arr(i64, [16]),
             arr(vp, [16]), arr(i64, [4096]), arr(vp, [16]), arr(i64, [4096]), arr(vp, [16]),
