@@ -306,6 +306,12 @@ int pass16bm_plan(const PassArgs &a)
     const int nch = (a.k1 + 31) >> 5;
     if (ts == 32 ? nch - (nch >> 1) > kLpC : nch > kLpC) return 0;
     if (a.raw0 < a.ncols && a.raw0 % 16 != 0) return 0;
+    // The FP32 loads move 16-byte groups of four rows (leading dimension a
+    // multiple of 4, 16-byte aligned slab); FP32 and FP64 loads use 32-bit
+    // per-lane byte offsets of up to 4 columns.
+    if (a.ldX32 % 4 != 0 || (uintptr_t)a.X32 % 16 != 0 || a.ldX32 > (int64_t)1 << 27 ||
+        a.ldX > (int64_t)1 << 27)
+        return 0;
     return ts;
 }
 

@@ -127,44 +127,95 @@ __global__ __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(
     const int64_t lxs = (int64_t)LSTEP * a.ldX;
     const int lpl = lcol * LD + lrow;
     typename std::conditional<V2, dbl2v, double>::type pf[NPF];
-    const float *lx32 = nullptr;
-    int64_t lxs32 = 0;
+    // Raw (FP32) columns: a.raw0 is a multiple of 16 and a wave's lanes move
+    // at most four consecutive columns of a 4-aligned group, so whether load
+    // step i is raw is uniform per wave.  A raw step moves RF = 2 VR FP32
+    // rows per lane (the prefetch register's width) with the first half of
+    // each column's lanes; the second half repeats the first half's rows and
+    // is not stored.  Its bits stay in the prefetch register and are
+    // converted (exactly) when the tile is written to LDS, so the load is
+    // one instruction of the FP64 step's width at a per-wave selected
+    // address and nothing waits for its data at issue.  The FP32 columns are
+    // addressed from a wave-uniform base (the wave's first load column) plus
+    // one 32-bit byte offset per lane, below 16 ldX32 bytes (the host checks
+    // the size and 16-byte alignment).
+    typedef float flt2v __attribute__((ext_vector_type(2)));
+    typedef float flt4v __attribute__((ext_vector_type(4)));
+    constexpr int RF = 2 * VR;
+    const int lq = lrow / VR, rq = lq & (TS / RF - 1);
+    const bool rstore = lq < TS / RF;
+    auto raw_step = [&](int i) {
+        return MX != 0 && __builtin_amdgcn_readfirstlane((int)(lcol + i * LSTEP >= a.raw0)) != 0;
+    };
+    // The mixed form addresses its FP64 columns the same way, so a load step
+    // keeps no 64-bit lane address; row checks compare against the
+    // wave-uniform count of rows left in the super-tile.
+    const char *wx32 = nullptr, *wx64 = nullptr;
+    int64_t lxs32 = 0, lxs64 = 0;
+    uint32_t lb32 = 0, lb64 = 0;
     if constexpr (MX != 0) {
-        lx32 = a.X32 + lrow + (int64_t)lcol * a.ldX32;
-        lxs32 = (int64_t)LSTEP * a.ldX32;
+        const int wcol = __builtin_amdgcn_readfirstlane(lcol);
+        wx32 = (const char *)(a.X32 + (int64_t)wcol * a.ldX32);
+        wx64 = (const char *)(a.X + (int64_t)wcol * a.ldX);
+        lxs32 = (int64_t)LSTEP * a.ldX32 * 4;
+        lxs64 = (int64_t)LSTEP * a.ldX * 8;
+        lb32 = 4u * (uint32_t)(RF * rq + (lcol - wcol) * a.ldX32);
+        lb64 = 8u * (uint32_t)(lrow + (lcol - wcol) * a.ldX);
     }
     auto load = [&](int64_t r0) {
-        const double *q = lx + r0;
-        if constexpr (V2) {
+        if constexpr (MX != 0) {
+            const int rem = (int)min(re - r0, (int64_t)TS);
+            // The lane offsets are opaque to the optimizer here, so each
+            // address stays (uniform base) + (32-bit lane offset) instead of
+            // eight hoisted 64-bit lane addresses per source.
+            uint32_t o32 = lb32, o64 = lb64;
+            asm volatile("" : "+v"(o32), "+v"(o64));
+            const char *b32 = wx32 + 4 * r0, *b64 = wx64 + 8 * r0;
+            const bool ok1 = lrow < rem, ok2 = lrow + VR - 1 < rem, ok4 = RF * rq + RF - 1 < rem;
+#pragma unroll
+            for (int i = 0; i < NPF; i++)
+                if (i < lnld) {
+                    using PV = typename std::remove_reference<decltype(pf[0])>::type;
+                    if (raw_step(i)) {
+                        const float *q32 = (const float *)(b32 + i * lxs32 + o32);
+                        if (ok4) {
+                            pf[i] = __builtin_nontemporal_load((const PV *)q32);
+                        } else {
+                            float v[RF];
+#pragma unroll
+                            for (int e = 0; e < RF; e++) v[e] = RF * rq + e < rem ? q32[e] : 0.f;
+                            if constexpr (V2)
+                                pf[i] = __builtin_bit_cast(dbl2v, flt4v{v[0], v[1], v[2], v[3]});
+                            else
+                                pf[i] = __builtin_bit_cast(double, flt2v{v[0], v[1]});
+                        }
+                    } else {
+                        const double *q64 = (const double *)(b64 + i * lxs64 + o64);
+                        if (ok2)
+                            pf[i] = __builtin_nontemporal_load((const PV *)q64);
+                        else if constexpr (V2)
+                            pf[i] = dbl2v{ok1 ? q64[0] : 0.0, 0.0};
+                        else
+                            pf[i] = 0.0;
+                    }
+                }
+        } else if constexpr (V2) {
+            const double *q = lx + r0;
             const bool ok2 = r0 + lrow + 1 < re, ok1 = r0 + lrow < re;
 #pragma unroll
             for (int i = 0; i < NPF; i++)
                 if (i < lnld) {
-                    // Raw columns: a.raw0 is a multiple of 16 and a wave's
-                    // lanes move at most four consecutive columns (a
-                    // 4-aligned group), so the choice is uniform per wave.
-                    if (MX != 0 &&
-                        __builtin_amdgcn_readfirstlane((int)(lcol + i * LSTEP >= a.raw0))) {
-                        // Raw column: two FP32 values, converted exactly.
-                        const float *q32 = lx32 + r0 + i * lxs32;
-                        pf[i] = dbl2v{ok1 ? (double)__builtin_nontemporal_load(q32) : 0.0,
-                                      ok2 ? (double)__builtin_nontemporal_load(q32 + 1) : 0.0};
-                    } else if (ok2)
+                    if (ok2)
                         pf[i] = __builtin_nontemporal_load((const dbl2v *)(q + i * lxs));
                     else
                         pf[i] = dbl2v{ok1 ? q[i * lxs] : 0.0, 0.0};
                 }
         } else {
+            const double *q = lx + r0;
             const bool ok = r0 + lrow < re;
 #pragma unroll
             for (int i = 0; i < NPF; i++)
-                if (i < lnld) {
-                    if (MX != 0 &&
-                        __builtin_amdgcn_readfirstlane((int)(lcol + i * LSTEP >= a.raw0)))
-                        pf[i] = ok ? (double)__builtin_nontemporal_load(lx32 + r0 + i * lxs32) : 0.0;
-                    else
-                        pf[i] = ok ? __builtin_nontemporal_load(q + i * lxs) : 0.0;
-                }
+                if (i < lnld) pf[i] = ok ? __builtin_nontemporal_load(q + i * lxs) : 0.0;
         }
     };
     if (rb < re) load(rb);
@@ -174,7 +225,20 @@ __global__ __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(
 #pragma unroll
         for (int i = 0; i < NPF; i++)
             if (i < lnld) {
-                if constexpr (V2) {
+                if (raw_step(i)) {
+                    if (rstore) {
+                        double *d = sm + lcol * LD + RF * rq + i * LSTEP * LD;
+                        if constexpr (V2) {
+                            const flt4v v = __builtin_bit_cast(flt4v, pf[i]);
+#pragma unroll
+                            for (int e = 0; e < 4; e++) d[e] = (double)v[e];
+                        } else {
+                            const flt2v v = __builtin_bit_cast(flt2v, pf[i]);
+                            d[0] = (double)v.x;
+                            d[1] = (double)v.y;
+                        }
+                    }
+                } else if constexpr (V2) {
                     sm[lpl + i * LSTEP * LD] = pf[i].x;
                     sm[lpl + 1 + i * LSTEP * LD] = pf[i].y;
                 } else {
@@ -389,8 +453,10 @@ __global__ __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(
 #pragma unroll
                 for (int j = 0; j < MT; j++)
                     acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ac[j], bc, acc[j], 0, 0, 0);
-                // Bound the operand loads in flight (register pressure).
-                if ((ks & 1) == 1) __builtin_amdgcn_sched_barrier(0);
+                // Bound the operand loads in flight (register pressure): two
+                // k-steps, one in the mixed form's four-tile instances (their
+                // raw-column loader holds a few more registers).
+                if ((MX != 0 && MT >= 4) || (ks & 1) == 1) __builtin_amdgcn_sched_barrier(0);
             }
         }
     }
