@@ -605,6 +605,9 @@ class Run {
     // largest coefficient block: (n + s) x n1.
     bool fused_fits() const
     {
+        // The generic fused pass addresses a column pair with 32-bit lane
+        // offsets.
+        if (ldX_ >= ((int64_t)1 << 28)) return false;
         PassArgs pa;
         memset(&pa, 0, sizeof(pa));
         pa.ncols = (int)n_;

@@ -662,26 +662,37 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
     const int nload = a.ncols * kPT;
     double pf[PF];
     // Element e of a tile is column e / 32, row e % 32: each half-wave reads
-    // 32 consecutive rows (256 bytes) of one column.
+    // 32 consecutive rows (256 bytes) of one column.  Step i of wave w reads
+    // columns 2 w + 8 i and 2 w + 8 i + 1, addressed from a wave-uniform
+    // base plus one 32-bit byte offset per lane (below 8 ldX + 256 bytes;
+    // the host keeps ldX under 2^28), so no per-step 64-bit lane address is
+    // kept.  Every lane loads unconditionally (one code path, so no
+    // register copies or zero defaults wait for loads or stores in flight):
+    // lanes past the last row re-read the last valid row, and the LDS store
+    // zeroes them.
+    const int lrow = tid & (kPT - 1);
+    const uint32_t lb = 8u * (uint32_t)((lane >> 5) * a.ldX + lrow);
+    const int64_t wcolb = (int64_t)(2 * wave) * a.ldX * 8, cstep = (int64_t)8 * a.ldX * 8;
     auto load = [&](int64_t r0) {
+        const int rem = (int)min(re - r0, (int64_t)kPT);
+        // Opaque to the optimizer, so the addresses are not hoisted.
+        uint32_t o = lb - 8u * (uint32_t)max(0, lrow - (rem - 1));
+        asm volatile("" : "+v"(o));
+        const char *b = (const char *)(a.X + r0) + wcolb;
 #pragma unroll
-        for (int i = 0; i < PF; i++) {
-            const int e = tid + 256 * i;
-            if (e < nload) {
-                const int64_t row = r0 + (e & (kPT - 1));
-                pf[i] = row < re ? a.X[row + (int64_t)(e / kPT) * a.ldX] : 0.0;
-            }
-        }
+        for (int i = 0; i < PF; i++)
+            if (tid + 256 * i < nload) pf[i] = *(const double *)(b + i * cstep + o);
     };
     if (rb < re) load(rb);
     __syncthreads();
 
     for (int64_t r0 = rb; r0 < re; r0 += kPT) {
         __syncthreads();
+        const bool rok = lrow < min(re - r0, (int64_t)kPT);
 #pragma unroll
         for (int i = 0; i < PF; i++) {
             const int e = tid + 256 * i;
-            if (e < nload) tile[(e / kPT) * kPLd + (e & (kPT - 1))] = pf[i];
+            if (e < nload) tile[(e / kPT) * kPLd + (e & (kPT - 1))] = rok ? pf[i] : 0.0;
         }
         __syncthreads();
         if (r0 + kPT < re) load(r0 + kPT);
