@@ -27,20 +27,27 @@ def say(*a):
     print(*a, flush=True)
 
 
-def mixed(X, s, precision, variant="bcgsi2P1s"):
-    """Run the mixed form on X rounded to FP32; returns (Q, R, status)."""
+def mixed(X, s, precision, variant="bcgsi2P1s", ld=None, pads=None):
+    """Run the mixed form on X rounded to FP32; returns (Q, R, status).
+    With ld > m, X32 and Q get leading dimension ld and NaN pad rows; the
+    pad rows of both after the call are appended to pads when given."""
     m, n = X.shape
+    ld = m if ld is None else ld
     X32 = np.asfortranarray(X.astype(np.float32))
-    x32 = torch.from_numpy(np.ascontiguousarray(X32.T)).cuda()
-    q = torch.zeros((n, m), dtype=torch.float64, device="cuda")
+    x32 = torch.full((n, ld), float("nan"), dtype=torch.float32)
+    x32[:, :m] = torch.from_numpy(np.ascontiguousarray(X32.T))
+    x32 = x32.cuda()
+    q = torch.full((n, ld), float("nan"), dtype=torch.float64, device="cuda")
     r = torch.zeros(n * n, dtype=torch.float64, device="cuda")
-    B.bcgs_mixed_device(variant, precision, m, n, s, x32.data_ptr(), m, q.data_ptr(), m,
+    B.bcgs_mixed_device(variant, precision, m, n, s, x32.data_ptr(), ld, q.data_ptr(), ld,
                         r.data_ptr(), n)
     B.sync()
     torch.cuda.synchronize()
     st = B.status()
-    Q = q.cpu().numpy().T
+    Q = q[:, :m].cpu().numpy().T
     R = r.cpu().numpy().reshape((n, n), order="F")
+    if pads is not None:
+        pads.append((x32[:, m:].cpu().numpy(), q[:, m:].cpu().numpy()))
     return Q, R, st, X32.astype(np.float64)
 
 
@@ -105,6 +112,33 @@ def test_reduced_precision_first_pass(precision, kernel, monkeypatch):
     assert 0.1 * lvl <= res <= 10 * lvl, (res, lvl)
     # R's upper triangle stays close to the FP64 factorization at that level.
     assert np.abs(R - Ro).max() <= 100 * lvl * np.abs(Ro).max()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "bf16"])
+@pytest.mark.parametrize("shape", [(5001, 48, 5004), (4099, 128, 4100), (301, 256, 304),
+                                   (20003, 64, 20004)])
+def test_padded_leading_dimension(shape, precision):
+    """Row counts that are not a multiple of 4 inside a leading dimension
+    that is: the 512-thread pass moves raw FP32 columns four rows per lane,
+    so the last group straddles the end of the rows.  Results match the
+    oracle (FP64 products) or keep the reduced-precision bounds (bf16), and
+    the NaN pad rows of X32 and Q are neither read into the result nor
+    written."""
+    m, n, ld = shape
+    X = I.svd_matrix(m, n, 1e2, seed=44)
+    say("case mixed padded", shape, precision)
+    pads = []
+    Q, R, st, Xr = mixed(X, 16, precision, ld=ld, pads=pads)
+    assert st["hip_error"] == 0 and st["info_block"] == 0
+    assert np.all(np.isnan(pads[0][0])) and np.all(np.isnan(pads[0][1]))
+    assert np.all(np.isfinite(Q)) and np.all(np.isfinite(R))
+    Qo, Ro, info = O.bcgs("bcgsi2P1s", Xr, 16)
+    if precision == "fp64":
+        assert np.abs(R - Ro).max() <= 1e-11 * np.abs(Ro).max()
+        assert np.abs(Q - Qo).max() <= 1e-10
+    else:
+        assert I.loo(Q) <= 10 * max(I.loo(Qo), 1e-14)
+        assert I.rel_residual(Xr, Q, R) <= 10 * LEVEL[precision]
 
 
 def test_mixed_rejects_unsupported():
