@@ -78,6 +78,23 @@ def test_batch_matches_oracle(variant):
                                                             1e-15), i
 
 
+@pytest.mark.parametrize("variant", ["bcgsi21s", "bcgspipi2", "bcgsi2P2s"])
+def test_batch_other_variants(variant):
+    """The remaining variants through the batched form: 12 uneven instances
+    (s = 4, 8, 16; n up to 128), each against the oracle as a single call."""
+    batch = Batch(make_batch(12, 56, kappa_hi=1e4), 56)
+    say("case batch", variant)
+    info, cfg = batch.run(variant)
+    for i, (m, n, s, kappa) in enumerate(batch.shapes):
+        Qo, Ro, info_o = O.bcgs(variant, batch.X[i], s)
+        assert info[i] == info_o, (i, info[i], info_o)
+        if info_o != (0, 0):
+            continue
+        Q, R = batch.result(i)
+        assert np.abs(R - Ro).max() <= 1e-11 * np.abs(Ro).max() * max(1.0, kappa / 1e3), i
+        assert I.loo(Q) <= 10 * max(I.loo(Qo), 1e-14), i
+
+
 def test_replay_is_bitwise_identical():
     batch = Batch(make_batch(12, 52), 52)
     say("case batch replay")
