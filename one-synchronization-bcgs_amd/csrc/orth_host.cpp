@@ -605,9 +605,9 @@ class Run {
     // largest coefficient block: (n + s) x n1.
     bool fused_fits() const
     {
-        // The generic fused pass addresses a column pair with 32-bit lane
-        // offsets.
-        if (ldX_ >= ((int64_t)1 << 28)) return false;
+        // The generic fused pass addresses up to four columns with 32-bit
+        // lane offsets.
+        if (ldX_ >= ((int64_t)1 << 27)) return false;
         PassArgs pa;
         memset(&pa, 0, sizeof(pa));
         pa.ncols = (int)n_;

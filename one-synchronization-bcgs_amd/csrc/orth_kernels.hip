@@ -624,8 +624,10 @@ size_t pass_lds_bytes(const PassArgs &a)
 constexpr int pass_waves(int S, int MT) { return MT > 4 || S > 16 ? 1 : S <= 8 ? 3 : 2; }
 
 // MT: projection tiles per wave, PF: prefetch registers per thread.  The
-// narrow instance (fewer of both) fits several workgroups per CU.
-template <int S, int MT = kPassMaxTiles, int PF = kPF>
+// narrow instance (fewer of both) fits several workgroups per CU.  V2: every
+// lane moves two consecutive rows with one 16-byte load (even ldX and m,
+// 16-byte aligned slab).
+template <int S, int MT = kPassMaxTiles, int PF = kPF, bool V2 = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
     pass_waves(S, MT), pass_waves(S, MT)))) void k_pass(PassArgs a)
 {
@@ -659,29 +661,34 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
 
     const int64_t rb = (int64_t)blockIdx.x * a.rows_per;
     const int64_t re = min(a.m, rb + a.rows_per);
-    const int nload = a.ncols * kPT;
-    double pf[PF];
-    // Element e of a tile is column e / 32, row e % 32: each half-wave reads
-    // 32 consecutive rows (256 bytes) of one column.  Step i of wave w reads
-    // columns 2 w + 8 i and 2 w + 8 i + 1, addressed from a wave-uniform
-    // base plus one 32-bit byte offset per lane (below 8 ldX + 256 bytes;
-    // the host keeps ldX under 2^28), so no per-step 64-bit lane address is
-    // kept.  Every lane loads unconditionally (one code path, so no
-    // register copies or zero defaults wait for loads or stores in flight):
-    // lanes past the last row re-read the last valid row, and the LDS store
-    // zeroes them.
-    const int lrow = tid & (kPT - 1);
-    const uint32_t lb = 8u * (uint32_t)((lane >> 5) * a.ldX + lrow);
-    const int64_t wcolb = (int64_t)(2 * wave) * a.ldX * 8, cstep = (int64_t)8 * a.ldX * 8;
+    // Load step i of wave w moves columns CPW (w + 4 i) .. + CPW - 1 (LPC
+    // lanes per column, VR consecutive rows per lane, so each group of LPC
+    // lanes reads 32 consecutive rows = 256 bytes of one column), addressed
+    // from a wave-uniform base plus one 32-bit byte offset per lane (below
+    // 8 (3 ldX + 32) bytes; the host keeps ldX under 2^27), so no per-step
+    // 64-bit lane address is kept.  Every lane loads unconditionally (one
+    // code path, so no register copies or zero defaults wait for loads or
+    // stores in flight): lanes past the last row re-read the last valid
+    // row(s), and the LDS store zeroes them.  With V2 the row count of every
+    // tile is even.
+    typedef double dbl2v __attribute__((ext_vector_type(2)));
+    using PV = typename std::conditional<V2, dbl2v, double>::type;
+    constexpr int VR = V2 ? 2 : 1, NPF = PF / VR;
+    constexpr int LPC = kPT / VR, CPW = 64 / LPC, CPS = 4 * CPW;
+    PV pf[NPF];
+    const int lrow = VR * (lane & (LPC - 1)), lcolw = lane / LPC;
+    const int lcol = CPW * wave + lcolw;
+    const uint32_t lb = 8u * (uint32_t)(lcolw * a.ldX + lrow);
+    const int64_t wcolb = (int64_t)(CPW * wave) * a.ldX * 8, cstep = (int64_t)CPS * a.ldX * 8;
     auto load = [&](int64_t r0) {
         const int rem = (int)min(re - r0, (int64_t)kPT);
         // Opaque to the optimizer, so the addresses are not hoisted.
-        uint32_t o = lb - 8u * (uint32_t)max(0, lrow - (rem - 1));
+        uint32_t o = lb - 8u * (uint32_t)max(0, lrow - (rem - VR));
         asm volatile("" : "+v"(o));
         const char *b = (const char *)(a.X + r0) + wcolb;
 #pragma unroll
-        for (int i = 0; i < PF; i++)
-            if (tid + 256 * i < nload) pf[i] = *(const double *)(b + i * cstep + o);
+        for (int i = 0; i < NPF; i++)
+            if (lcol + CPS * i < a.ncols) pf[i] = *(const PV *)(b + i * cstep + o);
     };
     if (rb < re) load(rb);
     __syncthreads();
@@ -690,10 +697,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
         __syncthreads();
         const bool rok = lrow < min(re - r0, (int64_t)kPT);
 #pragma unroll
-        for (int i = 0; i < PF; i++) {
-            const int e = tid + 256 * i;
-            if (e < nload) tile[(e / kPT) * kPLd + (e & (kPT - 1))] = rok ? pf[i] : 0.0;
-        }
+        for (int i = 0; i < NPF; i++)
+            if (lcol + CPS * i < a.ncols) {
+                double *d = tile + (lcol + CPS * i) * kPLd + lrow;
+                if constexpr (V2) {
+                    d[0] = rok ? pf[i].x : 0.0;
+                    d[1] = rok ? pf[i].y : 0.0;
+                } else {
+                    d[0] = rok ? pf[i] : 0.0;
+                }
+            }
         __syncthreads();
         if (r0 + kPT < re) load(r0 + kPT);
 
@@ -1164,19 +1177,27 @@ bool pass_narrow(const PassArgs &a)
     return a.ncols <= kNarrowCols && (a.na == 0 || Ma * Nb <= 4 * 4);
 }
 
-template <int S, int MT, int PF>
+template <int S, int MT, int PF, bool V2 = false>
 hipError_t launch_pass_v(const PassArgs &a, size_t lds, int nwg, hipStream_t st)
 {
     static bool attr_set = false;
     if (!attr_set) {
-        hipError_t e = hipFuncSetAttribute((const void *)k_pass<S, MT, PF>,
+        hipError_t e = hipFuncSetAttribute((const void *)k_pass<S, MT, PF, V2>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize,
                                            (int)(160 * 1024));
         if (e != hipSuccess) return e;
         attr_set = true;
     }
-    k_pass<S, MT, PF><<<dim3(nwg), dim3(256), lds, st>>>(a);
+    k_pass<S, MT, PF, V2><<<dim3(nwg), dim3(256), lds, st>>>(a);
     return hipGetLastError();
+}
+
+// Two-row (16-byte) loads: even leading dimension and row count, 16-byte
+// aligned slab.
+bool pass_v2_ok(const PassArgs &a)
+{
+    return a.ldX % 2 == 0 && a.m % 2 == 0 && (uintptr_t)a.X % 16 == 0 &&
+           !getenv("ORTH_PASS_LOAD8");
 }
 
 template <int S>
@@ -1184,7 +1205,9 @@ hipError_t launch_pass_t(const PassArgs &a, int nwg, hipStream_t st)
 {
     const size_t lds = pass_lds_bytes(a);
     if (lds == 0) return hipErrorInvalidValue;
-    if (pass_narrow(a)) return launch_pass_v<S, 4, kNarrowPF>(a, lds, nwg, st);
+    if (pass_narrow(a))
+        return pass_v2_ok(a) ? launch_pass_v<S, 4, kNarrowPF, true>(a, lds, nwg, st)
+                             : launch_pass_v<S, 4, kNarrowPF>(a, lds, nwg, st);
     return launch_pass_v<S, kPassMaxTiles, kPF>(a, lds, nwg, st);
 }
 

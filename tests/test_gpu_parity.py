@@ -276,14 +276,17 @@ def test_glued_wide(variant):
 
 
 @pytest.mark.skipif(torch is None, reason="torch missing")
+@pytest.mark.parametrize("s", [16, 8, 4])
 @pytest.mark.parametrize("variant", ["bcgsi2P1s", "bcgsi2"])
-@pytest.mark.parametrize("m,ld", [(4097, 4098), (4097, 4097), (6001, 6004)])
-def test_padded_leading_dimension(variant, m, ld):
+@pytest.mark.parametrize("m,ld", [(4097, 4098), (4097, 4097), (6001, 6004), (4098, 4102)])
+def test_padded_leading_dimension(variant, m, ld, s):
     """Odd row counts inside an even leading dimension (two-row loads with a
-    last row that has no partner) and odd leading dimensions (one-row loads)."""
-    n, s = 48, 16
+    last row that has no partner), odd leading dimensions (one-row loads) and
+    even row counts with pad rows (two-row loads throughout), for the s = 16
+    pass and the generic pass (s = 8, 4); the pad rows stay untouched."""
+    n = 48
     X = I.svd_matrix(m, n, 1e3, seed=34)
-    say("case padded ld", variant, m, ld)
+    say("case padded ld", variant, m, ld, s)
     Xp = np.zeros((ld, n))
     Xp[:m] = X
     Xd = torch.from_numpy(np.ascontiguousarray(Xp.T)).cuda()
