@@ -553,6 +553,7 @@ class Run {
             const int64_t mtp = ((na + 15) / 16 + 3) / 4;
             pa.wl = (int)std::max<int64_t>(pa.wl, a0 - cb + 64 * mtp);
         }
+        if (!mx_ && ts16 == 0) unit = pass_rows(S_, pa);
         // Two 512-thread workgroups per CU when the s = 16 pass applies (its
         // LDS is under half a CU's), so one hides the other's barriers.
         // The mixed pass with reduced-precision products runs one 512-thread

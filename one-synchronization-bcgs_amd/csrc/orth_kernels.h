@@ -152,10 +152,12 @@ constexpr int kPassRows = 32;
 constexpr int kPassMaxCols = 288;
 
 // LDS bytes the fused pass needs, or 0 when the shape does not fit it.
-size_t pass_lds_bytes(const PassArgs &a);
+size_t pass_lds_bytes(const PassArgs &a, int F = 1);
 hipError_t launch_pass(int S, const PassArgs &a, int nwg, hipStream_t st);
 // Workgroups per CU the generic pass for this shape can keep resident.
 int pass_wg_per_cu(int S, const PassArgs &a);
+// Rows per tile (the row-chunk unit) of the 256-thread passes for a.
+int pass_rows(int S, const PassArgs &a);
 
 // Block width 16 with 512 threads per workgroup: super-tile rows for the
 // pass (multiple of 32, 0 when it does not apply); rows_per must be a

@@ -602,12 +602,12 @@ constexpr int kPF = kPassMaxCols * kPT / 256;    // prefetch registers per threa
 constexpr int kPassMaxTiles = 12;                // projection tiles per wave
 constexpr int kPassMaxC = 8192;                  // doubles of C held in LDS
 
-size_t pass_lds_bytes(const PassArgs &a)
+size_t pass_lds_bytes(const PassArgs &a, int F)
 {
     if (a.ncols > kPassMaxCols || a.wl < a.ncols || a.wl > kPassMaxCols + 32) return 0;
     const int k1p = (a.k1 + 3) & ~3;
     if ((size_t)k1p * a.n1 > (size_t)kPassMaxC || k1p > a.wl) return 0;
-    if (a.k1 > 0 && (a.n1 % 16 != 0 || (kPT / 16) * (a.n1 / 16) > 8)) return 0;
+    if (a.k1 > 0 && (a.n1 % 16 != 0 || (F * kPT / 16) * (a.n1 / 16) > 8)) return 0;
     if (a.na > 0) {
         const int Ma = (a.na + 15) / 16, Nb = (a.nb + 15) / 16;
         if (Ma * Nb > 4 * kPassMaxTiles) return 0;
@@ -615,7 +615,7 @@ size_t pass_lds_bytes(const PassArgs &a)
     }
     for (int g = 0; g < a.ngrp; g++)
         if (a.g[g].w > kMaxS || a.g[g].t0 + a.g[g].w > a.ncols) return 0;
-    return ((size_t)a.wl * kPLd + (size_t)k1p * a.n1 + 2 * kMaxS * kMaxS) * sizeof(double);
+    return ((size_t)a.wl * (F * kPT + 1) + (size_t)k1p * a.n1 + 2 * kMaxS * kMaxS) * sizeof(double);
 }
 
 // Waves per SIMD (= 256-thread workgroups per CU) the generic pass is
@@ -626,17 +626,19 @@ constexpr int pass_waves(int S, int MT) { return MT > 4 || S > 16 ? 1 : S <= 8 ?
 // MT: projection tiles per wave, PF: prefetch registers per thread.  The
 // narrow instance (fewer of both) fits several workgroups per CU.  V2: every
 // lane moves two consecutive rows with one 16-byte load (even ldX and m,
-// 16-byte aligned slab).
-template <int S, int MT = kPassMaxTiles, int PF = kPF, bool V2 = false>
+// 16-byte aligned slab).  F: rows per tile = 32 F (the narrow instance with
+// few columns takes longer runs of each column per tile).
+template <int S, int MT = kPassMaxTiles, int PF = kPF, bool V2 = false, int F = 1>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
     pass_waves(S, MT), pass_waves(S, MT)))) void k_pass(PassArgs a)
 {
+    constexpr int TS = kPT * F, PLD = TS + 1;
     extern __shared__ double sm[];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int li = lane & 15, lk = lane >> 4;
     const int k1p = (a.k1 + 3) & ~3;
     double *tile = sm;
-    double *Cs = tile + a.wl * kPLd;
+    double *Cs = tile + a.wl * PLD;
     double *Ts = Cs + k1p * a.n1;  // one S x S factor per group, ld S
 
     for (int e = tid; e < k1p * a.n1; e += 256) Cs[e] = a.C[e];
@@ -649,8 +651,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
                 Ts[g * S * S + e] = (i < w && j < w) ? T[i + j * kLdT] : 0.0;
             }
     }
-    for (int e = tid; e < (a.wl - a.ncols) * kPT; e += 256)
-        tile[(a.ncols + e / kPT) * kPLd + (e % kPT)] = 0.0;
+    for (int e = tid; e < (a.wl - a.ncols) * TS; e += 256)
+        tile[(a.ncols + e / TS) * PLD + (e % TS)] = 0.0;
 
     const int Ma = (a.na + 15) >> 4, Nb = (a.nb + 15) >> 4;
     const int ntiles = a.na > 0 ? Ma * Nb : 0;
@@ -661,11 +663,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
 
     const int64_t rb = (int64_t)blockIdx.x * a.rows_per;
     const int64_t re = min(a.m, rb + a.rows_per);
-    // Load step i of wave w moves columns CPW (w + 4 i) .. + CPW - 1 (LPC
-    // lanes per column, VR consecutive rows per lane, so each group of LPC
-    // lanes reads 32 consecutive rows = 256 bytes of one column), addressed
-    // from a wave-uniform base plus one 32-bit byte offset per lane (below
-    // 8 (3 ldX + 32) bytes; the host keeps ldX under 2^27), so no per-step
+    // Load step i moves columns lcol + CPS i: LPC = TS / VR lanes per column,
+    // VR consecutive rows per lane, so each group of LPC lanes reads TS
+    // consecutive rows of one column.  A wave's lanes touch at most four
+    // consecutive columns; the loads are addressed from a wave-uniform base
+    // (the wave's first column) plus one 32-bit byte offset per lane (below
+    // 8 (3 ldX + TS) bytes; the host keeps ldX under 2^27), so no per-step
     // 64-bit lane address is kept.  Every lane loads unconditionally (one
     // code path, so no register copies or zero defaults wait for loads or
     // stores in flight): lanes past the last row re-read the last valid
@@ -674,14 +677,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
     typedef double dbl2v __attribute__((ext_vector_type(2)));
     using PV = typename std::conditional<V2, dbl2v, double>::type;
     constexpr int VR = V2 ? 2 : 1, NPF = PF / VR;
-    constexpr int LPC = kPT / VR, CPW = 64 / LPC, CPS = 4 * CPW;
+    constexpr int LPC = TS / VR, CPS = 256 / LPC;
+    static_assert(LPC >= 16 && LPC <= 256 && 256 % LPC == 0, "lane mapping");
     PV pf[NPF];
-    const int lrow = VR * (lane & (LPC - 1)), lcolw = lane / LPC;
-    const int lcol = CPW * wave + lcolw;
-    const uint32_t lb = 8u * (uint32_t)(lcolw * a.ldX + lrow);
-    const int64_t wcolb = (int64_t)(CPW * wave) * a.ldX * 8, cstep = (int64_t)CPS * a.ldX * 8;
+    const int lrow = VR * (tid % LPC), lcol = tid / LPC;
+    const int wcol = __builtin_amdgcn_readfirstlane(lcol);
+    const uint32_t lb = 8u * (uint32_t)((lcol - wcol) * a.ldX + lrow);
+    const int64_t wcolb = (int64_t)wcol * a.ldX * 8, cstep = (int64_t)CPS * a.ldX * 8;
     auto load = [&](int64_t r0) {
-        const int rem = (int)min(re - r0, (int64_t)kPT);
+        const int rem = (int)min(re - r0, (int64_t)TS);
         // Opaque to the optimizer, so the addresses are not hoisted.
         uint32_t o = lb - 8u * (uint32_t)max(0, lrow - (rem - VR));
         asm volatile("" : "+v"(o));
@@ -693,13 +697,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
     if (rb < re) load(rb);
     __syncthreads();
 
-    for (int64_t r0 = rb; r0 < re; r0 += kPT) {
+    for (int64_t r0 = rb; r0 < re; r0 += TS) {
         __syncthreads();
-        const bool rok = lrow < min(re - r0, (int64_t)kPT);
+        const bool rok = lrow < min(re - r0, (int64_t)TS);
 #pragma unroll
         for (int i = 0; i < NPF; i++)
             if (lcol + CPS * i < a.ncols) {
-                double *d = tile + (lcol + CPS * i) * kPLd + lrow;
+                double *d = tile + (lcol + CPS * i) * PLD + lrow;
                 if constexpr (V2) {
                     d[0] = rok ? pf[i].x : 0.0;
                     d[1] = rok ? pf[i].y : 0.0;
@@ -708,7 +712,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
                 }
             }
         __syncthreads();
-        if (r0 + kPT < re) load(r0 + kPT);
+        if (r0 + TS < re) load(r0 + TS);
 
         if (a.ngrp > 0) {
             // Step 1 in one phase (all groups from the same product) or, in
@@ -723,7 +727,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
                 }
                 const int ntlo = zlo >> 4, nN = ((zhi + 15) >> 4) - ntlo;
                 const int kp = a.seq ? ((a.g[p].k + 3) & ~3) : k1p;
-                const int ncombo = a.k1 > 0 ? (kPT / 16) * nN : 0;
+                const int ncombo = a.k1 > 0 ? (TS / 16) * nN : 0;
                 // Z = tile(:, 0:kp) C(:, N-tiles), one 16 x 16 tile of Z per combo.
                 dbl4 z[2];
 #pragma unroll
@@ -732,11 +736,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
                     const int combo = wave + 4 * q;
                     if (combo < ncombo) {
                         const int mt = combo / nN, nt = ntlo + combo % nN;
-                        const double *ap = tile + lk * kPLd + mt * 16 + li;
+                        const double *ap = tile + lk * PLD + mt * 16 + li;
                         const double *bp = Cs + lk * a.n1 + nt * 16 + li;
                         for (int ks = 0; ks < kp / 4; ks++)
                             z[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(
-                                ap[4 * ks * kPLd], bp[4 * ks * a.n1], z[q], 0, 0, 0);
+                                ap[4 * ks * PLD], bp[4 * ks * a.n1], z[q], 0, 0, 0);
                     }
                 }
                 __syncthreads();
@@ -750,7 +754,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
                         for (int g = glo; g < ghi; g++) {
                             const int off = zc - a.g[g].zoff;
                             if (off >= 0 && off < a.g[g].w) {
-                                double *col = tile + (a.g[g].t0 + off) * kPLd + mt * 16 + lk;
+                                double *col = tile + (a.g[g].t0 + off) * PLD + mt * 16 + lk;
 #pragma unroll
                                 for (int v = 0; v < 4; v++) col[4 * v] -= z[q][v];
                             }
@@ -760,15 +764,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
                 __syncthreads();
                 // Row-wise forward substitution with each group's factor
                 // (dtrsm_('R','U','N','N') on the rows of the tile).
-                for (int task = tid; task < (ghi - glo) * kPT; task += 256) {
-                    const int g = glo + task / kPT, row = task % kPT;
+                for (int task = tid; task < (ghi - glo) * TS; task += 256) {
+                    const int g = glo + task / TS, row = task % TS;
                     if (a.g[g].T) {
                         const int w = a.g[g].w;
-                        double *col = tile + a.g[g].t0 * kPLd + row;
+                        double *col = tile + a.g[g].t0 * PLD + row;
                         const double *Tg = Ts + g * S * S;
                         double x[S];
 #pragma unroll
-                        for (int i = 0; i < S; i++) x[i] = i < w ? col[i * kPLd] : 0.0;
+                        for (int i = 0; i < S; i++) x[i] = i < w ? col[i * PLD] : 0.0;
 #pragma unroll
                         for (int i = 0; i < S; i++) {
                             if (i < w) {
@@ -780,7 +784,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
                         }
 #pragma unroll
                         for (int i = 0; i < S; i++)
-                            if (i < w) col[i * kPLd] = x[i];
+                            if (i < w) col[i * PLD] = x[i];
                     }
                 }
                 __syncthreads();
@@ -788,10 +792,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
             // Write the finished target blocks back to X.
             for (int g = 0; g < a.ngrp; g++) {
                 const int t0 = a.g[g].t0;
-                for (int e = tid; e < a.g[g].w * kPT; e += 256) {
-                    const int cc = e / kPT, rr = e - cc * kPT;
+                for (int e = tid; e < a.g[g].w * TS; e += 256) {
+                    const int cc = e / TS, rr = e - cc * TS;
                     const int64_t row = r0 + rr;
-                    if (row < re) a.X[row + (int64_t)(t0 + cc) * a.ldX] = tile[(t0 + cc) * kPLd + rr];
+                    if (row < re) a.X[row + (int64_t)(t0 + cc) * a.ldX] = tile[(t0 + cc) * PLD + rr];
                 }
             }
         }
@@ -799,15 +803,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
         // Step 2: projection partials.
         if (ntiles > 0) {
 #pragma unroll
-            for (int ks = 0; ks < kPT / 4; ks++) {
+            for (int ks = 0; ks < TS / 4; ks++) {
                 const int rr = 4 * ks + lk;
 #pragma unroll
                 for (int t = 0; t < MT; t++) {
                     const int tl = wave + 4 * t;
                     if (tl < ntiles) {
                         const int ta = tl / Nb, tb = tl - ta * Nb;
-                        const double av = tile[(a.a0 + ta * 16 + li) * kPLd + rr];
-                        const double bv = tile[(a.b0 + tb * 16 + li) * kPLd + rr];
+                        const double av = tile[(a.a0 + ta * 16 + li) * PLD + rr];
+                        const double bv = tile[(a.b0 + tb * 16 + li) * PLD + rr];
                         acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[t], 0, 0, 0);
                     }
                 }
@@ -1177,19 +1181,36 @@ bool pass_narrow(const PassArgs &a)
     return a.ncols <= kNarrowCols && (a.na == 0 || Ma * Nb <= 4 * 4);
 }
 
-template <int S, int MT, int PF, bool V2 = false>
+template <int S, int MT, int PF, bool V2 = false, int F = 1>
 hipError_t launch_pass_v(const PassArgs &a, size_t lds, int nwg, hipStream_t st)
 {
     static bool attr_set = false;
     if (!attr_set) {
-        hipError_t e = hipFuncSetAttribute((const void *)k_pass<S, MT, PF, V2>,
+        hipError_t e = hipFuncSetAttribute((const void *)k_pass<S, MT, PF, V2, F>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize,
                                            (int)(160 * 1024));
         if (e != hipSuccess) return e;
         attr_set = true;
     }
-    k_pass<S, MT, PF, V2><<<dim3(nwg), dim3(256), lds, st>>>(a);
+    k_pass<S, MT, PF, V2, F><<<dim3(nwg), dim3(256), lds, st>>>(a);
     return hipGetLastError();
+}
+
+// Rows per tile of the generic pass, 32 F: the narrow instance takes F = 4
+// (S <= 8) or 2 (S <= 16) when the tile's columns fit its prefetch
+// registers (ncols * 32 F <= 256 * kNarrowPF) and the step-1 tiles fit.
+int pass_f(int S, const PassArgs &a)
+{
+    if (!pass_narrow(a) || getenv("ORTH_PASS_TILE32")) return 1;
+    for (int F = S <= 8 ? 4 : S <= 16 ? 2 : 1; F > 1; F >>= 1)
+        if (a.ncols * kPT * F <= 256 * kNarrowPF && pass_lds_bytes(a, F) != 0) return F;
+    return 1;
+}
+
+int pass_rows(int S, const PassArgs &a)
+{
+    if (S == 16 && pass16_mt(a) > 0 && !getenv("ORTH_GENERIC_PASS")) return kPT;  // k_pass16
+    return kPT * pass_f(S, a);
 }
 
 // Two-row (16-byte) loads: even leading dimension and row count, 16-byte
@@ -1200,20 +1221,32 @@ bool pass_v2_ok(const PassArgs &a)
            !getenv("ORTH_PASS_LOAD8");
 }
 
+template <int S, int F>
+hipError_t launch_pass_n(const PassArgs &a, size_t lds, int nwg, hipStream_t st)
+{
+    return pass_v2_ok(a) ? launch_pass_v<S, 4, kNarrowPF, true, F>(a, lds, nwg, st)
+                         : launch_pass_v<S, 4, kNarrowPF, false, F>(a, lds, nwg, st);
+}
+
 template <int S>
 hipError_t launch_pass_t(const PassArgs &a, int nwg, hipStream_t st)
 {
-    const size_t lds = pass_lds_bytes(a);
-    if (lds == 0) return hipErrorInvalidValue;
-    if (pass_narrow(a))
-        return pass_v2_ok(a) ? launch_pass_v<S, 4, kNarrowPF, true>(a, lds, nwg, st)
-                             : launch_pass_v<S, 4, kNarrowPF>(a, lds, nwg, st);
+    const int F = pass_f(S, a);
+    const size_t lds = pass_lds_bytes(a, F);
+    if (lds == 0 || a.rows_per % (kPT * F) != 0) return hipErrorInvalidValue;
+    if (pass_narrow(a)) {
+        if constexpr (S <= 8)
+            if (F == 4) return launch_pass_n<S, 4>(a, lds, nwg, st);
+        if constexpr (S <= 16)
+            if (F == 2) return launch_pass_n<S, 2>(a, lds, nwg, st);
+        return launch_pass_n<S, 1>(a, lds, nwg, st);
+    }
     return launch_pass_v<S, kPassMaxTiles, kPF>(a, lds, nwg, st);
 }
 
 int pass_wg_per_cu(int S, const PassArgs &a)
 {
-    const size_t lds = pass_lds_bytes(a);
+    const size_t lds = pass_lds_bytes(a, pass_f(S, a));
     if (lds == 0) return 1;
     // Register-limited residency (k_pass16 and the wide generic pass: one
     // workgroup per CU).
