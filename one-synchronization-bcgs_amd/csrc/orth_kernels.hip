@@ -107,7 +107,7 @@ hipError_t launch_update(int S, double *X, int64_t ldX, int64_t m,
 constexpr int kRT = 16;       // rows per LDS tile
 constexpr int kLdsLd = kRT + 1;  // padded column stride in LDS (doubles)
 constexpr int kMaxTilesPerWave = 12;
-constexpr int kMaxLdsCols = 30 * 16;
+constexpr int kMaxLdsCols = 20 * 16;
 
 ProjPlan plan_proj(int64_t m, int64_t na, int64_t nb, int target_wg)
 {
@@ -136,6 +136,9 @@ ProjPlan plan_proj(int64_t m, int64_t na, int64_t nb, int target_wg)
     return p;
 }
 
+// PF: load steps per tile (16 columns each), at least the column group's
+// width / 16; every step loads (columns past the group read column a0).
+template <int PF>
 __global__ __launch_bounds__(256) void k_proj(const double *__restrict__ X, int64_t ldX,
                                               int64_t m, int64_t a0, int na, int64_t b0,
                                               int nb, int Ma, int Nb, int ga, int gb,
@@ -157,37 +160,68 @@ __global__ __launch_bounds__(256) void k_proj(const double *__restrict__ X, int6
 
     const int64_t rb = (int64_t)blockIdx.x * rows_per;
     const int64_t re = min(m, rb + rows_per);
+    // The next tile is prefetched into registers while the current one is
+    // multiplied.  Element tid + 256 i of a tile is column (tid >> 4) + 16 i,
+    // row tid & 15, so load step i of wave w moves columns 4 w + 16 i .. + 3,
+    // which never straddle the A / B boundary (colsA is a multiple of 16):
+    // the step's first column is wave-uniform and each lane adds its column
+    // offset (lane >> 4).  Every lane loads unconditionally from a valid
+    // address (columns past na / nb read column a0, rows past the end the
+    // last row); the LDS store zeroes those elements.
+    const int rr = threadIdx.x & (kRT - 1), lc = lane >> 4;
+    const int64_t lcoff = (int64_t)lc * ldX;
+    double pf[PF];
+    // Group column cl (wave-uniform cw = cl - lc) -> matrix column and validity.
+    auto colbase = [&](int cw, int64_t &base, bool &lok) {
+        if (cw < colsA) {
+            const int ac = ta0 * 16 + cw;
+            lok = ac + lc < na;
+            base = ac < na ? a0 + ac : a0;
+        } else {
+            const int bc = tb0 * 16 + (cw - colsA);
+            lok = bc + lc < nb;
+            base = bc < nb ? b0 + bc : a0;
+        }
+    };
+    auto load = [&](int64_t r0) {
+        int64_t rowc = min(r0 + rr, re - 1);
+        // Opaque to the optimizer, so the addresses are not hoisted.
+        asm volatile("" : "+v"(rowc));
+#pragma unroll
+        for (int i = 0; i < PF; i++) {
+            const int cw = 4 * wave + 16 * i;
+            int64_t base = a0;
+            bool lok = false;
+            if (cw < ncol) colbase(cw, base, lok);
+            pf[i] = X[base * ldX + (lok ? lcoff : 0) + rowc];
+        }
+    };
+    if (rb < re) load(rb);
     for (int64_t r0 = rb; r0 < re; r0 += kRT) {
         __syncthreads();
-        for (int e = threadIdx.x; e < ncol * kRT; e += 256) {
-            const int cl = e / kRT, rr = e - cl * kRT;
-            const int64_t row = r0 + rr;
-            int64_t gcol;
-            bool ok;
-            if (cl < colsA) {
-                const int ac = ta0 * 16 + cl;
-                ok = ac < na;
-                gcol = a0 + ac;
-            } else {
-                const int bc = tb0 * 16 + (cl - colsA);
-                ok = bc < nb;
-                gcol = b0 + bc;
+        const bool rok = r0 + rr < re;
+#pragma unroll
+        for (int i = 0; i < PF; i++) {
+            const int cw = 4 * wave + 16 * i;
+            if (cw < ncol) {
+                int64_t base;
+                bool lok;
+                colbase(cw, base, lok);
+                lds[(cw + lc) * kLdsLd + rr] = lok && rok ? pf[i] : 0.0;
             }
-            double v = 0.0;
-            if (ok && row < re) v = X[row + gcol * ldX];
-            lds[cl * kLdsLd + rr] = v;
         }
         __syncthreads();
+        if (r0 + kRT < re) load(r0 + kRT);
 #pragma unroll
         for (int ks = 0; ks < kRT / 4; ks++) {
-            const int rr = ks * 4 + lk;
+            const int rk = ks * 4 + lk;
 #pragma unroll
             for (int t = 0; t < kMaxTilesPerWave; t++) {
                 const int tile = wave + 4 * t;
                 if (tile < ntiles) {
                     const int ta = tile / ntb, tb = tile - ta * ntb;
-                    const double a = lds[(ta * 16 + li) * kLdsLd + rr];
-                    const double b = lds[(colsA + tb * 16 + li) * kLdsLd + rr];
+                    const double a = lds[(ta * 16 + li) * kLdsLd + rk];
+                    const double b = lds[(colsA + tb * 16 + li) * kLdsLd + rk];
                     acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[t], 0, 0, 0);
                 }
             }
@@ -216,8 +250,18 @@ hipError_t launch_proj(const ProjPlan &pl, const double *X, int64_t ldX, int64_t
                        hipStream_t st)
 {
     dim3 grid(pl.nwg, pl.ngy), block(256);
-    k_proj<<<grid, block, pl.lds_bytes, st>>>(X, ldX, m, a0, (int)na, b0, (int)nb, pl.Ma,
-                                              pl.Nb, pl.ga, pl.gb, pl.rows_per, part);
+    const int steps = (pl.ga + pl.gb + 3) / 4 * 4;  // 16-column load steps, rounded up to 4
+#define ORTH_PROJ(P)                                                                        \
+    case P:                                                                                 \
+        k_proj<P><<<grid, block, pl.lds_bytes, st>>>(X, ldX, m, a0, (int)na, b0, (int)nb,   \
+                                                     pl.Ma, pl.Nb, pl.ga, pl.gb,            \
+                                                     pl.rows_per, part);                    \
+        break;
+    switch (steps) {
+        ORTH_PROJ(4) ORTH_PROJ(8) ORTH_PROJ(12) ORTH_PROJ(16) ORTH_PROJ(20)
+    default: return hipErrorInvalidValue;
+    }
+#undef ORTH_PROJ
     return hipGetLastError();
 }
 
