@@ -84,10 +84,185 @@ __global__ __launch_bounds__(256) void k_update(double *X, int64_t ldX, int64_t 
     if (nupd > 1) apply_upd<S>(xr, ldX, u1);
 }
 
+// Block updates on FP64 MFMA (s = 16, 32: NT = s / 16 column tiles).  Each
+// wave owns 16 MT rows.  The products over the shared prefix [0, u0.c) of
+// both updates run in one K-loop (one read of the prefix); the triangular
+// solve X <- X T^{-1} is the blocked scheme of the s = 16 pass (one MFMA per
+// 4-column block with the row blocks [D_b | -D_b T(b, >)] the Cholesky step
+// stores beside T).  u1's prefix is u0's prefix followed by u0's block
+// (checked by the host): that block's final values go from registers
+// through LDS into u1's product.  No workgroup barrier: waves past the last
+// row return.
+template <int NT, int MT>
+__global__ __launch_bounds__(256) void k_upd(double *X, int64_t ldX, int64_t m, Upd u0, Upd u1,
+                                             int nupd)
+{
+    constexpr int S = 16 * NT, LDY = S + 1, RW = 16 * MT;
+    __shared__ double ys_all[4][RW * LDY];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+    const int64_t row0 = ((int64_t)blockIdx.x * 4 + wave) * RW;
+    if (row0 >= m) return;
+    double *ys = ys_all[wave];
+    const bool two = nupd > 1;
+    int64_t ra[MT];  // A-operand row of this lane per M-tile (clamped to the slab)
+#pragma unroll
+    for (int t = 0; t < MT; t++) ra[t] = min(row0 + 16 * t + li, m - 1);
+    dbl4 z0[MT][NT], z1[MT][NT];
+#pragma unroll
+    for (int t = 0; t < MT; t++)
+#pragma unroll
+        for (int n = 0; n < NT; n++) {
+            z0[t][n] = dbl4{0.0, 0.0, 0.0, 0.0};
+            z1[t][n] = dbl4{0.0, 0.0, 0.0, 0.0};
+        }
+    // Z += X(:, j0:j0+4) W(j0:j0+4, :) for j0 in [lo, hi), for u0 and (when
+    // paired) u1: A lane (li, lk) holds X(row li, j0 + lk), B lane holds
+    // W(j0 + lk, 16 n + li); columns past hi read column hi - 1 with zero
+    // coefficients.
+    auto kloop = [&](int lo, int hi) {
+#pragma unroll 4
+        for (int j0 = lo; j0 < hi; j0 += 4) {
+            const int j = j0 + lk;
+            const bool jok = j < hi;
+            const int64_t jc = jok ? j : hi - 1;
+            double a[MT];
+#pragma unroll
+            for (int t = 0; t < MT; t++) a[t] = X[ra[t] + jc * ldX];
+#pragma unroll
+            for (int n = 0; n < NT; n++) {
+                const int col = 16 * n + li;
+                {
+                    const double b = jok && col < u0.w ? u0.W[j + (int64_t)col * u0.ldW] : 0.0;
+#pragma unroll
+                    for (int t = 0; t < MT; t++)
+                        z0[t][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[t], b, z0[t][n], 0, 0, 0);
+                }
+                if (two) {
+                    const double b = jok && col < u1.w ? u1.W[j + (int64_t)col * u1.ldW] : 0.0;
+#pragma unroll
+                    for (int t = 0; t < MT; t++)
+                        z1[t][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[t], b, z1[t][n], 0, 0, 0);
+                }
+            }
+        }
+    };
+    // Y = X(:, col0:col0+w) - Z, then Y <- Y T^{-1}, then store (D layout:
+    // lane (li, lk) value v is row lk + 4 v, column 16 n + li).
+    auto finish = [&](const Upd &u, dbl4 (&z)[MT][NT]) {
+#pragma unroll
+        for (int t = 0; t < MT; t++)
+#pragma unroll
+            for (int n = 0; n < NT; n++) {
+                const int col = 16 * n + li;
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const int64_t row = row0 + 16 * t + lk + 4 * v;
+                    const double x = row < m && col < u.w ? X[row + (u.col0 + col) * ldX] : 0.0;
+                    z[t][n][v] = x - z[t][n][v];
+                }
+            }
+        if (u.T) {
+            const double *T = u.T;
+            for (int b = 0; b < (u.w + 3) >> 2; b++) {
+                const int nt = b >> 2, cb = b & 3;
+                // Block b's current values go through LDS to become A operands.
+                if ((li >> 2) == cb) {
+#pragma unroll
+                    for (int t = 0; t < MT; t++)
+#pragma unroll
+                        for (int n = 0; n < NT; n++)
+                            if (n == nt) {
+#pragma unroll
+                                for (int v = 0; v < 4; v++)
+                                    ys[(16 * t + lk + 4 * v) * LDY + 4 * b + (li & 3)] = z[t][n][v];
+                            }
+                }
+                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+                double av[MT];
+#pragma unroll
+                for (int t = 0; t < MT; t++) av[t] = ys[(16 * t + li) * LDY + 4 * b + lk];
+#pragma unroll
+                for (int n = 0; n < NT; n++) {
+                    if (n < nt) continue;
+                    const double bv = T[kSlotW + 128 * b + lk * kLdT + 16 * n + li];
+#pragma unroll
+                    for (int t = 0; t < MT; t++) {
+                        if (n == nt && (li >> 2) == cb) z[t][n] = dbl4{0.0, 0.0, 0.0, 0.0};
+                        z[t][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[t], bv, z[t][n], 0, 0, 0);
+                    }
+                }
+                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < MT; t++)
+#pragma unroll
+            for (int n = 0; n < NT; n++) {
+                const int col = 16 * n + li;
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const int64_t row = row0 + 16 * t + lk + 4 * v;
+                    if (row < m && col < u.w) X[row + (u.col0 + col) * ldX] = z[t][n][v];
+                }
+            }
+    };
+    if (u0.c > 0) kloop(0, u0.c);
+    finish(u0, z0);
+    if (two) {
+        // u1 += (u0's finished block) W1(u0.c : u0.c + u0.w, :).
+#pragma unroll
+        for (int t = 0; t < MT; t++)
+#pragma unroll
+            for (int n = 0; n < NT; n++)
+                if (16 * n + li < u0.w) {
+#pragma unroll
+                    for (int v = 0; v < 4; v++)
+                        ys[(16 * t + lk + 4 * v) * LDY + 16 * n + li] = z0[t][n][v];
+                }
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        for (int j0 = 0; j0 < u0.w; j0 += 4) {
+            const int j = j0 + lk;
+            const bool jok = j < u0.w;
+            double av[MT];
+#pragma unroll
+            for (int t = 0; t < MT; t++) av[t] = jok ? ys[(16 * t + li) * LDY + j] : 0.0;
+#pragma unroll
+            for (int n = 0; n < NT; n++) {
+                const int col = 16 * n + li;
+                const double b = jok && col < u1.w ? u1.W[u0.c + j + (int64_t)col * u1.ldW] : 0.0;
+#pragma unroll
+                for (int t = 0; t < MT; t++)
+                    z1[t][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[t], b, z1[t][n], 0, 0, 0);
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        finish(u1, z1);
+    }
+}
+
+template <int NT>
+hipError_t launch_upd(double *X, int64_t ldX, int64_t m, const Upd *u, int nupd, hipStream_t st)
+{
+    constexpr int MT = 2;
+    const Upd u1 = nupd > 1 ? u[1] : u[0];
+    const int64_t rows = 4 * 16 * MT;
+    k_upd<NT, MT><<<dim3((unsigned)((m + rows - 1) / rows)), dim3(256), 0, st>>>(X, ldX, m, u[0], u1,
+                                                                               nupd);
+    return hipGetLastError();
+}
+
 hipError_t launch_update(int S, double *X, int64_t ldX, int64_t m,
                          const Upd *u, int nupd, hipStream_t st)
 {
     if (m <= 0 || nupd <= 0) return hipSuccess;
+    // The MFMA form takes the schedules' pairing (u1's prefix is u0's prefix
+    // followed by u0's block), where it shares the prefix and coefficient
+    // loads between the two products; single updates stay row-wise (there
+    // the MFMA form's coefficient loads cost more than it saves).
+    const bool paired = nupd == 2 && u[0].col0 == u[0].c && u[1].c == u[0].c + u[0].w;
+    if ((S == 16 || S == 32) && paired && !getenv("ORTH_UPDATE_ROWWISE"))
+        return S == 16 ? launch_upd<1>(X, ldX, m, u, nupd, st)
+                       : launch_upd<2>(X, ldX, m, u, nupd, st);
     dim3 grid((unsigned)((m + 255) / 256)), block(256);
     Upd u1 = nupd > 1 ? u[1] : u[0];
     switch (S) {
