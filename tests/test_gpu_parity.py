@@ -204,12 +204,17 @@ def test_large_slab_properties(variant):
     assert res / xn < 1e-15
 
 
+@pytest.mark.parametrize("rowwise", [False, True])
 @pytest.mark.parametrize("variant", VARIANTS)
-@pytest.mark.parametrize("shape", [(4096, 64, 16), (3000, 37, 5), (2000, 96, 32)])
-def test_unfused_path_matches_oracle(variant, shape, monkeypatch):
+@pytest.mark.parametrize("shape", [(4096, 64, 16), (3000, 37, 5), (2000, 96, 32), (3001, 80, 16),
+                                   (2001, 96, 32)])
+def test_unfused_path_matches_oracle(variant, shape, rowwise, monkeypatch):
     """The separate update / projection kernels (used when a shape does not
-    fit the fused pass) stay at parity too."""
+    fit the fused pass) stay at parity too, with the paired updates on MFMA
+    (row counts that end inside a wave's 32 rows included) or row-wise."""
     monkeypatch.setenv("ORTH_UNFUSED", "1")
+    if rowwise:
+        monkeypatch.setenv("ORTH_UPDATE_ROWWISE", "1")
     m, n, s = shape
     X = I.svd_matrix(m, n, 1e4, seed=29)
     check_against_oracle(variant, X, s)
