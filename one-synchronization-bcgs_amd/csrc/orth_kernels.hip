@@ -93,9 +93,9 @@ __global__ __launch_bounds__(256) void k_update(double *X, int64_t ldX, int64_t 
 // (checked by the host): that block's final values go from registers
 // through LDS into u1's product.  No workgroup barrier: waves past the last
 // row return.
-template <int NT, int MT>
-__global__ __launch_bounds__(256) void k_upd(double *X, int64_t ldX, int64_t m, Upd u0, Upd u1,
-                                             int nupd)
+// TWO: u1 follows u0 (the pairing); else u0 alone.
+template <int NT, int MT, bool TWO>
+__global__ __launch_bounds__(256) void k_upd(double *X, int64_t ldX, int64_t m, Upd u0, Upd u1)
 {
     constexpr int S = 16 * NT, LDY = S + 1, RW = 16 * MT;
     __shared__ double ys_all[4][RW * LDY];
@@ -103,17 +103,17 @@ __global__ __launch_bounds__(256) void k_upd(double *X, int64_t ldX, int64_t m, 
     const int64_t row0 = ((int64_t)blockIdx.x * 4 + wave) * RW;
     if (row0 >= m) return;
     double *ys = ys_all[wave];
-    const bool two = nupd > 1;
+    constexpr bool two = TWO;
     int64_t ra[MT];  // A-operand row of this lane per M-tile (clamped to the slab)
 #pragma unroll
     for (int t = 0; t < MT; t++) ra[t] = min(row0 + 16 * t + li, m - 1);
-    dbl4 z0[MT][NT], z1[MT][NT];
+    dbl4 z0[MT][NT], z1[TWO ? MT : 1][NT];
 #pragma unroll
     for (int t = 0; t < MT; t++)
 #pragma unroll
         for (int n = 0; n < NT; n++) {
             z0[t][n] = dbl4{0.0, 0.0, 0.0, 0.0};
-            z1[t][n] = dbl4{0.0, 0.0, 0.0, 0.0};
+            if constexpr (TWO) z1[t][n] = dbl4{0.0, 0.0, 0.0, 0.0};
         }
     // Z += X(:, j0:j0+4) W(j0:j0+4, :) for j0 in [lo, hi), for u0 and (when
     // paired) u1: A lane (li, lk) holds X(row li, j0 + lk), B lane holds
@@ -137,7 +137,7 @@ __global__ __launch_bounds__(256) void k_upd(double *X, int64_t ldX, int64_t m, 
                     for (int t = 0; t < MT; t++)
                         z0[t][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[t], b, z0[t][n], 0, 0, 0);
                 }
-                if (two) {
+                if constexpr (TWO) {
                     const double b = jok && col < u1.w ? u1.W[j + (int64_t)col * u1.ldW] : 0.0;
 #pragma unroll
                     for (int t = 0; t < MT; t++)
@@ -148,7 +148,7 @@ __global__ __launch_bounds__(256) void k_upd(double *X, int64_t ldX, int64_t m, 
     };
     // Y = X(:, col0:col0+w) - Z, then Y <- Y T^{-1}, then store (D layout:
     // lane (li, lk) value v is row lk + 4 v, column 16 n + li).
-    auto finish = [&](const Upd &u, dbl4 (&z)[MT][NT]) {
+    auto finish = [&](const Upd &u, auto &z) {
 #pragma unroll
         for (int t = 0; t < MT; t++)
 #pragma unroll
@@ -208,7 +208,7 @@ __global__ __launch_bounds__(256) void k_upd(double *X, int64_t ldX, int64_t m, 
     };
     if (u0.c > 0) kloop(0, u0.c);
     finish(u0, z0);
-    if (two) {
+    if constexpr (TWO) {
         // u1 += (u0's finished block) W1(u0.c : u0.c + u0.w, :).
 #pragma unroll
         for (int t = 0; t < MT; t++)
@@ -240,14 +240,13 @@ __global__ __launch_bounds__(256) void k_upd(double *X, int64_t ldX, int64_t m, 
     }
 }
 
+// Paired updates, 32 rows per wave.
 template <int NT>
-hipError_t launch_upd(double *X, int64_t ldX, int64_t m, const Upd *u, int nupd, hipStream_t st)
+hipError_t launch_upd_pair(double *X, int64_t ldX, int64_t m, const Upd *u, hipStream_t st)
 {
-    constexpr int MT = 2;
-    const Upd u1 = nupd > 1 ? u[1] : u[0];
-    const int64_t rows = 4 * 16 * MT;
-    k_upd<NT, MT><<<dim3((unsigned)((m + rows - 1) / rows)), dim3(256), 0, st>>>(X, ldX, m, u[0], u1,
-                                                                               nupd);
+    const int64_t rows = 4 * 16 * 2;
+    k_upd<NT, 2, true><<<dim3((unsigned)((m + rows - 1) / rows)), dim3(256), 0, st>>>(X, ldX, m,
+                                                                                    u[0], u[1]);
     return hipGetLastError();
 }
 
@@ -257,12 +256,12 @@ hipError_t launch_update(int S, double *X, int64_t ldX, int64_t m,
     if (m <= 0 || nupd <= 0) return hipSuccess;
     // The MFMA form takes the schedules' pairing (u1's prefix is u0's prefix
     // followed by u0's block), where it shares the prefix and coefficient
-    // loads between the two products; single updates stay row-wise (there
-    // the MFMA form's coefficient loads cost more than it saves).
+    // loads between the two products; single updates stay row-wise (there,
+    // even at 64 rows per wave, the MFMA form's coefficient loads cost more
+    // than it saves).
     const bool paired = nupd == 2 && u[0].col0 == u[0].c && u[1].c == u[0].c + u[0].w;
     if ((S == 16 || S == 32) && paired && !getenv("ORTH_UPDATE_ROWWISE"))
-        return S == 16 ? launch_upd<1>(X, ldX, m, u, nupd, st)
-                       : launch_upd<2>(X, ldX, m, u, nupd, st);
+        return S == 16 ? launch_upd_pair<1>(X, ldX, m, u, st) : launch_upd_pair<2>(X, ldX, m, u, st);
     dim3 grid((unsigned)((m + 255) / 256)), block(256);
     Upd u1 = nupd > 1 ? u[1] : u[0];
     switch (S) {
