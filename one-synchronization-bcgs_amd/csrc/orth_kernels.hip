@@ -2,8 +2,13 @@
 //
 //  k_update  row-local block updates, one row per lane, coefficients read as
 //            wave-uniform (scalar) loads
-//  k_proj    A^T B over a row slab on v_mfma_f64_16x16x4f64, tiles staged
-//            through LDS; one partial per workgroup
+//  k_upd     the schedules' paired block updates on v_mfma_f64_16x16x4f64
+//            (shared prefix reads, blocked MFMA triangular solve)
+//  k_proj    A^T B over a row slab on v_mfma_f64_16x16x4f64, 16-row tiles
+//            staged through LDS with the next tile prefetched into
+//            registers; one partial per workgroup
+//  k_pass    the fused pass for any block width (256 threads; s = 16 mostly
+//            runs the 512-thread k_pass16b of orth_pass16b.h)
 //  k_reduce  fixed-order sum of the partials (the in-kernel replacement of
 //            the reference's MPI_Allreduce)
 //  k_small   the s x s / c x s steps between passes, one workgroup, FP64
