@@ -451,10 +451,20 @@ __global__ __launch_bounds__(256) void k_reduce(const double *__restrict__ part,
     __shared__ double red[8][32];
     const int el = threadIdx.x & 31, g = threadIdx.x >> 5;
     const int64_t e = (int64_t)blockIdx.x * 32 + el;
-    double s = 0.0;
-    if (e < E)
-        for (int w = g; w < nwg; w += 8) s += part[(int64_t)w * E + e];
-    red[g][el] = s;
+    // Four independent partial sums (four loads in flight per thread),
+    // combined in a fixed order: the result is the same on every run.
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    if (e < E) {
+        int w = g;
+        for (; w + 24 < nwg; w += 32) {
+            s0 += part[(int64_t)w * E + e];
+            s1 += part[(int64_t)(w + 8) * E + e];
+            s2 += part[(int64_t)(w + 16) * E + e];
+            s3 += part[(int64_t)(w + 24) * E + e];
+        }
+        for (; w < nwg; w += 8) s0 += part[(int64_t)w * E + e];
+    }
+    red[g][el] = (s0 + s1) + (s2 + s3);
     __syncthreads();
     if (g == 0 && e < E) {
         double t = 0.0;
