@@ -12,7 +12,7 @@ algorithmic byte count B_alg = 8 m (2n + s N(N-1)/2) of one factorization
 the HIP-event time of the call.  `roofline` describes the dominant kernel
 class, timed with HIP events around each of its launches on the stream it
 runs on; `cpu_baseline` times the CPU oracle (a C restatement of the
-reference) on a bounded sample of the same workload, scaled linearly in m.
+reference) on the same input on the host cores (all m rows by default).
 """
 import argparse
 import json
@@ -64,24 +64,31 @@ def f_alg(m, n):
 
 def make_input(torch, m, n, kappa, seed):
     """X^T as an n x m row-major tensor (= column-major m x n, ld m)."""
-    g = torch.Generator(device="cuda")
-    g.manual_seed(seed)
-    G = torch.randn((n, m), generator=g, dtype=torch.float64, device="cuda")
-    for _ in range(2):  # CholQR2: rows of G become orthonormal (U^T)
-        L = torch.linalg.cholesky(G @ G.T)
-        G = torch.linalg.inv(L) @ G
-    V, _ = torch.linalg.qr(torch.randn((n, n), generator=g, dtype=torch.float64,
-                                       device="cuda"))
-    sig = torch.logspace(0, -float(torch.log10(torch.tensor(kappa))), n,
-                         dtype=torch.float64, device="cuda")
-    X = (V * sig) @ G
-    del G
-    return X.contiguous()
+    import device_inputs
+
+    return device_inputs.svd_input(torch, m, n, kappa, seed)
+
+
+def host_cpu():
+    """(model name, logical CPUs visible) of this host."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        ncpu = len(os.sched_getaffinity(0))
+    except AttributeError:
+        ncpu = os.cpu_count() or 1
+    return model, ncpu
 
 
 def cpu_baseline(X0, variant, n, s, rows, threads, m_full):
     """Time the CPU oracle on the first `rows` rows of the device input X0
-    (the same bits the GPU factorizes), scaled linearly in m."""
+    (the same bits the GPU factorizes); scaled linearly in m when rows < m."""
     import inputs
     import oracle
 
@@ -90,18 +97,23 @@ def cpu_baseline(X0, variant, n, s, rows, threads, m_full):
     t0 = time.perf_counter()
     Q, R, info = oracle.bcgs(variant, X, s)
     t = time.perf_counter() - t0
-    loo = inputs.loo(Q)
+    loo = inputs.loo(Q) if rows <= (1 << 22) else None
+    del Q, R, X
     t_full = t * (m_full / rows)
+    model, ncpu = host_cpu()
+    scaled = "" if rows == m_full else ", scaled linearly in m to m=%d" % m_full
     return {
         "value": round(b_alg(m_full, n, s) / t_full / 1e9, 3),
         "unit": "GB/s",
         "cores": threads,
         "kind": "port",
-        "sample": ("CPU oracle (C restatement, OpenMP) %s FP64 on rows 0..%d of the same "
-                   "device input (n=%d, s=%d), %.2f s, scaled linearly in m to m=%d: "
-                   "%.1f ms per factorization; sample LOO %.1e" % (
-                       variant, rows, n, s, t, m_full, t_full * 1e3, loo)),
+        "sample": ("CPU oracle (C restatement of SRC/orth.c, OpenMP, %d threads) %s FP64 on "
+                   "rows 0..%d of the same device input (n=%d, s=%d), %.2f s%s: %.1f ms per "
+                   "factorization%s" % (
+                       threads, variant, rows, n, s, t, scaled, t_full * 1e3,
+                       "" if loo is None else "; sample LOO %.1e" % loo)),
         "ms_per_factorization": round(t_full * 1e3, 1),
+        "host": {"cpu_model": model, "logical_cpus_visible": ncpu, "threads_used": threads},
     }
 
 
@@ -121,20 +133,9 @@ def pass_traffic(variant, m, n, s):
 def make_glued(torch, m, n, s, delta, seed):
     """Blocks X_k = Q0 C_k + delta E_k (one fixed orthonormal m x s basis Q0),
     as an n x m row-major tensor (= column-major m x n)."""
-    g = torch.Generator(device="cuda")
-    g.manual_seed(seed)
-    Q0 = torch.randn((s, m), generator=g, dtype=torch.float64, device="cuda")
-    for _ in range(2):
-        L = torch.linalg.cholesky(Q0 @ Q0.T)
-        Q0 = torch.linalg.inv(L) @ Q0
-    X = torch.empty((n, m), dtype=torch.float64, device="cuda")
-    for k in range(n // s):
-        C = torch.randn((s, s), generator=g, dtype=torch.float64, device="cuda")
-        X[k * s:(k + 1) * s] = C.T @ Q0
-        X[k * s:(k + 1) * s] += delta * torch.randn((s, m), generator=g, dtype=torch.float64,
-                                                     device="cuda")
-    del Q0
-    return X
+    import device_inputs
+
+    return device_inputs.glued_input(torch, m, n, s, delta, seed)
 
 
 def run_one(torch, B, variant, X0, m, n, s, reps=3):
@@ -388,8 +389,12 @@ def main():
     ap.add_argument("--s", type=int, default=16)
     ap.add_argument("--kappa", type=float, default=1e6)
     ap.add_argument("--variant", default="bcgsi2P1s")
-    ap.add_argument("--cpu-rows", type=int, default=1 << 23)
-    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-rows", type=int, default=0,
+                    help="rows of the CPU baseline sample (default: all m rows)")
+    ap.add_argument("--cpu-threads", type=int,
+                    default=int(os.environ.get("OMP_NUM_THREADS", "16")),
+                    help="OpenMP threads of the CPU baseline (default: OMP_NUM_THREADS, the "
+                         "host share of this GPU, else 16)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-mixed", action="store_true",
@@ -403,7 +408,7 @@ def main():
                     help="run the per-variant / per-kappa / glued measurement cases "
                          "instead of the headline line (one JSON object per case)")
     ap.add_argument("--copy-probe", action="store_true",
-                    help="after timing, run one plain elementwise copy of X (read 8mn, "
+                    help="after timing, run one extra plain elementwise copy of X (read 8mn, "
                          "write 8mn bytes) to calibrate HBM byte counters")
     args = ap.parse_args()
 
@@ -473,6 +478,20 @@ def main():
         acc = {"loo": loo, "rel_residual": res / xn}
         log("LOO %.3e  residual %.3e" % (loo, res / xn))
 
+    # Streaming ceiling of this device: a plain copy X0 -> X (after the
+    # accuracy check: it overwrites the factorization's Q) (8mn bytes read
+    # and 8mn written), best of three, timed with HIP events.
+    copy_ms = []
+    for _ in range(3):
+        c0, c1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        c0.record(stream)
+        X.copy_(X0)
+        c1.record(stream)
+        torch.cuda.synchronize()
+        copy_ms.append(c0.elapsed_time(c1))
+    copy_gbs = 16.0 * m * n / (min(copy_ms) / 1e3) / 1e9
+    log("copy X0 -> X: %.3f ms, %.1f GB/s" % (min(copy_ms), copy_gbs))
+
     # Mixed-precision leg: the same input rounded to FP32 storage; first-pass
     # products in each reduced precision, FP64 reorthogonalization.
     mixed = {}
@@ -506,23 +525,43 @@ def main():
             log("mixed %s: %s" % (prec, json.dumps(finite(d))))
         del X32
 
-    # Dominant kernel class and its roofline.
+    # Dominant kernel class and its roofline.  The fused pass carries every
+    # O(m) product of the factorization, so its algorithmic work per
+    # factorization is F_alg = 4 m n^2 flop and B_alg bytes; per launch that
+    # is F_alg (B_alg) over the launches of one factorization, and
+    # achieved = F_alg / (time of the class per factorization), i.e.
+    # frac = max(B_alg / BW, F_alg / P) / t.  The schedule's own flops (with
+    # the triangular solves and the partly used product tiles) and the
+    # counter-measured traffic are reported beside it.
     dom = max(kt.items(), key=lambda kv: kv[1]["ms"])
     name, d = dom
     sec = d["ms"] / 1e3
-    gbs = d["bytes"] / sec / 1e9
-    tfs = d["flops"] / sec / 1e12
-    if d["flops"] / (FP64_PEAK_TFLOPS * 1e12) >= d["bytes"] / (HBM_PEAK_GBS * 1e9):
-        roof = {"bound": "mfma", "achieved": round(tfs, 3), "peak": FP64_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": round(tfs / FP64_PEAK_TFLOPS, 4)}
+    sched_gbs = d["bytes"] / sec / 1e9
+    sched_tfs = d["flops"] / sec / 1e12
+    per_fact = sec / args.steps
+    launches_per_fact = d["launches"] / args.steps
+    if name == "pass":
+        fa, ba = f_alg(m, n), float(b_alg(m, n, s))
     else:
-        roof = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4)}
+        fa, ba = d["flops"] / args.steps, d["bytes"] / args.steps
+    t_f, t_b = fa / (FP64_PEAK_TFLOPS * 1e12), ba / (HBM_PEAK_GBS * 1e9)
+    if t_f >= t_b:
+        ach = fa / per_fact / 1e12
+        roof = {"bound": "mfma", "achieved": round(ach, 3), "peak": FP64_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": round(ach / FP64_PEAK_TFLOPS, 4)}
+    else:
+        ach = ba / per_fact / 1e9
+        roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4)}
     traffic = pass_traffic(args.variant, m, n, s) if name == "pass" else None
     roof.update({"traffic": traffic, "kernel": name, "launches": d["launches"],
                  "avg_launch_ms": round(d["ms"] / d["launches"], 4),
+                 "alg_flops_per_launch": fa / launches_per_fact,
+                 "alg_bytes_per_launch": ba / launches_per_fact,
                  "share_of_step": round(d["ms"] / (ms * args.steps), 3),
-                 "alg_gbs": round(gbs, 1), "alg_tflops": round(tfs, 3)})
+                 "alg_gbs": round(ba / per_fact / 1e9, 1),
+                 "schedule_tflops": round(sched_tfs, 3), "schedule_gbs": round(sched_gbs, 1),
+                 "copy_gbs": round(copy_gbs, 1)})
 
     value = b_alg(m, n, s) / (ms / 1e3) / 1e9
     out = {
@@ -554,8 +593,8 @@ def main():
         out["mixed"] = mixed
     if not args.no_cpu:
         log("CPU baseline ...")
-        out["cpu_baseline"] = cpu_baseline(X0, args.variant, n, s, min(args.cpu_rows, m),
-                                           args.cpu_threads, m)
+        rows = m if args.cpu_rows <= 0 else min(args.cpu_rows, m)
+        out["cpu_baseline"] = cpu_baseline(X0, args.variant, n, s, rows, args.cpu_threads, m)
     print(json.dumps(finite(out)), flush=True)
 
 

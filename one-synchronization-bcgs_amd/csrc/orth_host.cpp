@@ -24,6 +24,7 @@
 #include "../../include/orth.h"
 #include "../../include/orth_ext.h"
 #include "orth_kernels.h"
+#include "orth_scratch.h"
 
 namespace orth {
 namespace {
@@ -36,21 +37,10 @@ struct DevCtx {
     bool init = false;
     hipEvent_t done = nullptr;  // end of the last enqueued call
     bool done_valid = false;
-    double *part = nullptr;
-    size_t part_n = 0;
-    double *P[2] = {nullptr, nullptr};
-    size_t P_n = 0;
-    double *stash = nullptr;  // Sig | Om | Cq[3] | Ctmp
-    double *Tbig = nullptr;   // (n + kMaxS) x kMaxS scratch of the small steps
-    size_t Tbig_n = 0;
-    double *Cbuf = nullptr;   // coefficients of the next fused pass
-    size_t Cbuf_n = 0;
+    Scratch sc;
     int ncu = 0;
-    double *Rd = nullptr;
-    size_t Rd_n = 0;
     double *Xd = nullptr;
     size_t Xd_n = 0;
-    int *status_d = nullptr;
     int *status_h = nullptr;
     orth_status_t last = {0, 0, 0, 0, 0};
     int last_passes = 0, last_syncs = 0;
@@ -90,6 +80,17 @@ bool ensure(double *&p, size_t &cap, size_t n)
     return true;
 }
 
+// Grow one scratch buffer of sc to n doubles (fixed scratch: fail instead).
+bool grow(Scratch &sc, double *&p, size_t &cap, size_t n)
+{
+    if (cap >= n && p) return true;
+    if (sc.fixed) {
+        tl_hip_error = (int)hipErrorOutOfMemory;
+        return false;
+    }
+    return ensure(p, cap, n);
+}
+
 int current_device()
 {
     int d = 0;
@@ -101,8 +102,8 @@ bool ctx_init(DevCtx &c)
 {
     if (c.init) return true;
     if (!ok(hipEventCreateWithFlags(&c.done, hipEventDisableTiming))) return false;
-    if (!ok(hipMalloc(&c.stash, 6 * kSlot * sizeof(double)))) return false;
-    if (!ok(hipMalloc(&c.status_d, 4 * sizeof(int)))) return false;
+    if (!ok(hipMalloc(&c.sc.stash, scratch_stash_need() * sizeof(double)))) return false;
+    if (!ok(hipMalloc(&c.sc.status_d, 4 * sizeof(int)))) return false;
     if (!ok(hipHostMalloc(&c.status_h, 4 * sizeof(int), hipHostMallocDefault))) return false;
     int dev = 0;
     (void)hipGetDevice(&dev);
@@ -110,6 +111,19 @@ bool ctx_init(DevCtx &c)
         return false;
     c.init = true;
     return true;
+}
+
+// Environment switches that select kernels (alternate and fallback paths,
+// read at each call): part of the key of captured batch graphs.
+int64_t env_bits()
+{
+    static const char *const names[] = {"ORTH_UNFUSED", "ORTH_UPDATE_ROWWISE", "ORTH_PASS256",
+                                        "ORTH_GENERIC_PASS", "ORTH_MIXED_PASS32",
+                                        "ORTH_PASS_TILE32", "ORTH_PASS_LOAD8"};
+    int64_t b = 0;
+    for (int i = 0; i < (int)(sizeof(names) / sizeof(names[0])); i++)
+        if (getenv(names[i])) b |= (int64_t)1 << i;
+    return b;
 }
 
 int block_template(int64_t s)
@@ -145,14 +159,15 @@ bool timed(DevCtx &c, int kind, hipStream_t st, double flops, double bytes, F la
 // One orthogonalization call: the schedule of passes for one variant.
 class Run {
   public:
-    Run(DevCtx &c, int64_t m, int64_t n, int64_t s, double *X, int64_t ldX, hipStream_t st)
-        : c_(c), m_(m), n_(n), s_(s), X_(X), ldX_(ldX), st_(st)
+    Run(DevCtx &c, Scratch &sc, int64_t m, int64_t n, int64_t s, double *X, int64_t ldX,
+        hipStream_t st)
+        : c_(c), sc_(sc), m_(m), n_(n), s_(s), X_(X), ldX_(ldX), st_(st)
     {
         S_ = block_template(s);
         N_ = 1 + (n - s + s - 1) / s;
         n1_ = std::max(16, 2 * S_);
         ldR_ = n;
-        Sig_ = c.stash;
+        Sig_ = sc.stash;
         Om_ = Sig_ + kSlot;
         Cq_ = Om_ + kSlot;
         Ctmp_ = Cq_ + 3 * kSlot;
@@ -163,27 +178,35 @@ class Run {
         // Largest projection: (n + 3 s) x (3 s) covers every schedule; the
         // extra kMaxS columns keep coefficient reads past a ragged block in
         // bounds.
-        const size_t pmax = (size_t)(n_ + 3 * kMaxS) * (4 * kMaxS) + 64;
-        if (c_.P_n < pmax || !c_.P[0] || !c_.P[1]) {
-            for (int k = 0; k < 2; k++) {
-                if (c_.P[k]) (void)hipFree(c_.P[k]);
-                c_.P[k] = nullptr;
+        const size_t pmax = scratch_p_need(n_);
+        if (sc_.P_n < pmax || !sc_.P[0] || !sc_.P[1]) {
+            if (sc_.fixed) {
+                tl_hip_error = (int)hipErrorOutOfMemory;
+                return false;
             }
-            c_.P_n = 0;
-            if (!ok(hipMalloc(&c_.P[0], pmax * sizeof(double)))) return false;
-            if (!ok(hipMalloc(&c_.P[1], pmax * sizeof(double)))) return false;
-            c_.P_n = pmax;
+            for (int k = 0; k < 2; k++) {
+                if (sc_.P[k]) (void)hipFree(sc_.P[k]);
+                sc_.P[k] = nullptr;
+            }
+            sc_.P_n = 0;
+            if (!ok(hipMalloc(&sc_.P[0], pmax * sizeof(double)))) return false;
+            if (!ok(hipMalloc(&sc_.P[1], pmax * sizeof(double)))) return false;
+            sc_.P_n = pmax;
         }
-        const size_t partmax = (size_t)kProjTargetWG * (n_ + 3 * kMaxS) * (3 * kMaxS);
-        if (!ensure(c_.part, c_.part_n, partmax)) return false;
-        const size_t rmax = (size_t)n_ * (n_ + kMaxS) + 64;
-        if (!ensure(c_.Rd, c_.Rd_n, rmax)) return false;
-        if (!ensure(c_.Tbig, c_.Tbig_n, (size_t)(n_ + kMaxS) * kMaxS)) return false;
-        if (!ensure(c_.Cbuf, c_.Cbuf_n, (size_t)(n_ + 2 * kMaxS) * 2 * kMaxS)) return false;
-        if (!ok(launch_fill(c_.Rd, rmax, st_))) return false;
-        if (!ok(launch_fill(c_.stash, 6 * kSlot, st_))) return false;
-        if (!ok(launch_fill_int(c_.status_d, 4, st_))) return false;
-        Rd_ = c_.Rd;
+        // Growable scratch reserves the partials of the widest plan up front;
+        // fixed scratch is checked pass by pass.
+        if (!sc_.fixed) {
+            const size_t partmax = (size_t)kProjTargetWG * (n_ + 3 * kMaxS) * (3 * kMaxS);
+            if (!grow(sc_, sc_.part, sc_.part_n, partmax)) return false;
+        }
+        const size_t rmax = scratch_r_need(n_);
+        if (!grow(sc_, sc_.Rd, sc_.Rd_n, rmax)) return false;
+        if (!grow(sc_, sc_.Tbig, sc_.Tbig_n, scratch_tbig_need(n_))) return false;
+        if (!grow(sc_, sc_.Cbuf, sc_.Cbuf_n, scratch_cbuf_need(n_))) return false;
+        if (!ok(launch_fill(sc_.Rd, rmax, st_))) return false;
+        if (!ok(launch_fill(sc_.stash, scratch_stash_need(), st_))) return false;
+        if (!ok(launch_fill_int(sc_.status_d, 4, st_))) return false;
+        Rd_ = sc_.Rd;
         return true;
     }
 
@@ -193,22 +216,28 @@ class Run {
     double *proj(int64_t a0, int64_t na, int64_t b0, int64_t nb)
     {
         ProjPlan pl = plan_proj(m_, na, nb, std::max(1, (int)(kProjTargetWG * wg_scale_)));
+        if (sc_.fixed && (size_t)pl.nwg * na * nb > sc_.part_n) {
+            // Fixed scratch: fewer row chunks, so that the partials fit.
+            const int64_t cap = (int64_t)(sc_.part_n / ((size_t)na * nb));
+            if (cap < 1) return nullptr;
+            pl = plan_proj(m_, na, nb, (int)std::min<int64_t>(INT_MAX, cap * pl.ngy));
+        }
         const size_t need = (size_t)pl.nwg * na * nb;
-        if (!ensure(c_.part, c_.part_n, need)) return nullptr;
+        if (!grow(sc_, sc_.part, sc_.part_n, need)) return nullptr;
         if (pl.lds_bytes > 64 * 1024) return nullptr;
         // Algorithmic work: 2 flops per row and output entry; each column of
         // A union B read once.
         const int64_t lo = std::max(a0, b0), hi = std::min(a0 + na, b0 + nb);
         const int64_t ucols = na + nb - std::max<int64_t>(0, hi - lo);
         if (!timed(c_, ORTH_K_PROJ, st_, 2.0 * m_ * na * nb, 8.0 * m_ * ucols, [&] {
-                return launch_proj(pl, X_, ldX_, m_, a0, na, b0, nb, c_.part, st_);
+                return launch_proj(pl, X_, ldX_, m_, a0, na, b0, nb, sc_.part, st_);
             }))
             return nullptr;
         pcur_ ^= 1;
-        double *P = c_.P[pcur_];
+        double *P = sc_.P[pcur_];
         if (!timed(c_, ORTH_K_REDUCE, st_, (double)pl.nwg * na * nb,
                    8.0 * ((double)pl.nwg + 1) * na * nb,
-                   [&] { return launch_reduce(c_.part, pl.nwg, na, nb, P, na, st_); }))
+                   [&] { return launch_reduce(sc_.part, pl.nwg, na, nb, P, na, st_); }))
             return nullptr;
         passes_++;
         syncs_++;
@@ -224,11 +253,11 @@ class Run {
         a.Om = Om_;
         a.Cq = Cq_;
         a.Ctmp = Ctmp_;
-        a.status = c_.status_d;
-        a.Tbig = c_.Tbig;
+        a.status = sc_.status_d;
+        a.Tbig = sc_.Tbig;
         a.pshift = g_pshift.load();
         if (fused_) {
-            a.Cout = c_.Cbuf;
+            a.Cout = sc_.Cbuf;
             a.n1 = n1_;
             a.S = S_;
         }
@@ -500,7 +529,7 @@ class Run {
         pa.m = m_;
         pa.k1 = pend_.k1;
         pa.n1 = n1_;
-        pa.C = c_.Cbuf;
+        pa.C = sc_.Cbuf;
         pa.ngrp = pend_.ngrp;
         pa.seq = pend_.seq;
         for (int g = 0; g < pend_.ngrp; g++) {
@@ -524,7 +553,7 @@ class Run {
         pa.na = (int)na;
         pa.b0 = (int)(b0 - cb);
         pa.nb = (int)nb;
-        pa.part = c_.part;
+        pa.part = sc_.part;
         // Kernel choice: the 512-thread s = 16 pass when it applies, else the
         // 256-thread passes (whose s = 16 form needs the tile padded to
         // ceil(Ma / 4) projection tiles per wave).
@@ -563,11 +592,17 @@ class Run {
         // registers and LDS allow.
         const int per_cu = (ts16 > 0 || mx_ == kMxFP64) ? 2 : mx_ ? 1 : pass_wg_per_cu(S_, pa);
         int nwg = std::max(1, (int)(std::max(1, c_.ncu) * per_cu * wg_scale_));
+        if (na > 0 && sc_.fixed) {
+            // Fixed scratch: fewer row chunks, so that the partials fit.
+            const int64_t cap = (int64_t)(sc_.part_n / ((size_t)na * nb));
+            if (cap < 1) return nullptr;
+            nwg = (int)std::min<int64_t>(nwg, cap);
+        }
         int64_t rows = (m_ + nwg - 1) / nwg;
         rows = (rows + unit - 1) / unit * unit;
         pa.rows_per = rows;
         nwg = (int)((m_ + rows - 1) / rows);
-        if (na > 0 && !ensure(c_.part, c_.part_n, (size_t)nwg * na * nb)) return nullptr;
+        if (na > 0 && !grow(sc_, sc_.part, sc_.part_n, (size_t)nwg * na * nb)) return nullptr;
         double flops = 2.0 * m_ * na * nb, wcols = 0.0;
         for (int g = 0; g < pend_.ngrp; g++) {
             const double kk = pend_.seq ? pend_.g[g].k : pend_.k1;
@@ -586,11 +621,11 @@ class Run {
         lp_from_ = -1;
         pend_ = Pending();
         passes_++;
-        if (na == 0) return c_.P[pcur_];
+        if (na == 0) return sc_.P[pcur_];
         pcur_ ^= 1;
-        double *P = c_.P[pcur_];
+        double *P = sc_.P[pcur_];
         if (!timed(c_, ORTH_K_REDUCE, st_, (double)nwg * na * nb, 8.0 * ((double)nwg + 1) * na * nb,
-                   [&] { return launch_reduce(c_.part, nwg, na, nb, P, na, st_); }))
+                   [&] { return launch_reduce(sc_.part, nwg, na, nb, P, na, st_); }))
             return nullptr;
         syncs_++;
         return P;
@@ -816,9 +851,9 @@ class Run {
     {
         (void)hipStreamSynchronize(st_);
         std::vector<double> h(6 * kSlot);
-        (void)hipMemcpy(h.data(), c_.stash, h.size() * sizeof(double), hipMemcpyDeviceToHost);
+        (void)hipMemcpy(h.data(), sc_.stash, h.size() * sizeof(double), hipMemcpyDeviceToHost);
         int stat[4];
-        (void)hipMemcpy(stat, c_.status_d, sizeof(stat), hipMemcpyDeviceToHost);
+        (void)hipMemcpy(stat, sc_.status_d, sizeof(stat), hipMemcpyDeviceToHost);
         fprintf(stderr, "[orth] op=%d blk=%d c=%d wb=%d wn=%d na=%d status=%d/%d\n", a.op,
                 a.blk, a.c, a.wb, a.wn, a.na, stat[0], stat[1]);
         if (a.P) {
@@ -839,6 +874,7 @@ class Run {
 
   private:
     DevCtx &c_;
+    Scratch &sc_;
     int64_t m_, n_, s_;
     double *X_;
     int64_t ldX_;
@@ -877,12 +913,13 @@ int run_device(DevCtx &c, int variant, int64_t m, int64_t n, int64_t s, double *
                int mx = 0, const float *x32 = nullptr, int64_t ldx32 = 0)
 {
     if (c.done_valid && !ok(hipStreamWaitEvent(st, c.done, 0))) return 2;
-    Run r(c, m, n, s, dX, ldX, st);
+    Run r(c, c.sc, m, n, s, dX, ldX, st);
     if (mx) r.set_mixed(mx, x32, ldx32);
     if (!r.prepare()) return 2;
     if (!r.run(variant)) return 2;
     if (!r.copy_R_out(Rout, ldR, rkind)) return 2;
-    if (!ok(hipMemcpyAsync(c.status_h, c.status_d, 4 * sizeof(int), hipMemcpyDeviceToHost, st)))
+    if (!ok(hipMemcpyAsync(c.status_h, c.sc.status_d, 4 * sizeof(int), hipMemcpyDeviceToHost,
+                            st)))
         return 2;
     if (!ok(hipEventRecord(c.done, st))) return 2;
     c.done_valid = true;
@@ -969,6 +1006,7 @@ constexpr int64_t kMaxGraphNodes = 12000;
 
 struct BatchCtx {
     bool init = false;
+    bool broken = false;  // a stream stayed in capture mode: refuse further batches
     DevCtx slot[kMaxSlots];
     hipStream_t st[kMaxSlots] = {};
     hipStream_t main = nullptr;
@@ -979,7 +1017,7 @@ struct BatchCtx {
     size_t info_n = 0;
     std::vector<hipGraphExec_t> execs;  // the captured batch, in launch order
     std::vector<int64_t> key;           // batch the graphs were captured for
-    BatchCfg key_cfg;
+    std::vector<int64_t> nocap;         // batch whose capture failed (runs directly)
     std::vector<std::pair<std::vector<int64_t>, BatchCfg>> tuned;  // shape signature -> config
 };
 
@@ -1047,11 +1085,13 @@ std::vector<int64_t> full_key(const BatchArgs &a, const BatchCfg &cfg)
     k.push_back(cfg.graph ? 1 : 0);
     k.push_back(cfg.streams);
     k.push_back((int64_t)(cfg.wg_scale * 1e6));
-    // Captured graphs hold the small steps' arguments, the shift among them.
+    // Captured graphs hold the small steps' arguments, the shift among them,
+    // and the kernels the environment switches selected.
     const double ps = g_pshift.load();
     int64_t psb = 0;
     memcpy(&psb, &ps, sizeof(psb));
     k.push_back(psb);
+    k.push_back(env_bits());
     for (int i = 0; i < a.count; i++) {
         k.push_back((int64_t)(intptr_t)a.X[i]);
         k.push_back(a.ldX[i]);
@@ -1097,47 +1137,85 @@ bool ensure_info(BatchCtx &b, int count)
     return true;
 }
 
-// Size every slot's scratch for the widest instance of the batch before
-// anything is captured: a captured graph keeps the buffer addresses, so no
-// slot buffer may be reallocated while graphs of this batch exist.
-bool batch_reserve(BatchCtx &b, const BatchArgs &a, int streams)
+void drop_graphs(BatchCtx &b)
 {
-    int im = 0;
-    for (int i = 1; i < a.count; i++)
-        if (a.n[i] > a.n[im]) im = i;
-    for (int k = 0; k < streams; k++) {
-        Run r(b.slot[k], a.m[im], a.n[im], a.s[im], a.X[im], a.ldX[im], b.st[k]);
-        if (!r.prepare()) return false;
-    }
-    if (!ensure_info(b, a.count)) return false;
-    return ok(hipDeviceSynchronize());
+    for (hipGraphExec_t e : b.execs) (void)hipGraphExecDestroy(e);
+    b.execs.clear();
+    b.key.clear();
 }
 
-// Enqueue instances on b.main (fork, per-stream sequences, join); `last`
-// adds the copy of every instance's status to the host.
+// Size every slot's scratch for the widest instance of the batch.  Runs only
+// while no graph of the batch exists (a captured graph keeps the buffer
+// addresses); the batch's own runs then never reallocate (the scratch is
+// frozen while they are enqueued).
+bool batch_reserve(BatchCtx &b, const BatchArgs &a, int streams)
+{
+    int64_t nmax = 0;
+    for (int i = 0; i < a.count; i++) nmax = std::max(nmax, a.n[i]);
+    for (int k = 0; k < streams; k++) {
+        Scratch &sc = b.slot[k].sc;
+        const size_t pmax = scratch_p_need(nmax);
+        if (sc.P_n < pmax || !sc.P[0] || !sc.P[1]) {
+            for (int j = 0; j < 2; j++) {
+                if (sc.P[j]) (void)hipFree(sc.P[j]);
+                sc.P[j] = nullptr;
+            }
+            sc.P_n = 0;
+            if (!ok(hipMalloc(&sc.P[0], pmax * sizeof(double)))) return false;
+            if (!ok(hipMalloc(&sc.P[1], pmax * sizeof(double)))) return false;
+            sc.P_n = pmax;
+        }
+        const size_t partmax = (size_t)kProjTargetWG * (nmax + 3 * kMaxS) * (3 * kMaxS);
+        if (!grow(sc, sc.part, sc.part_n, partmax) || !grow(sc, sc.Rd, sc.Rd_n, scratch_r_need(nmax)) ||
+            !grow(sc, sc.Tbig, sc.Tbig_n, scratch_tbig_need(nmax)) ||
+            !grow(sc, sc.Cbuf, sc.Cbuf_n, scratch_cbuf_need(nmax)))
+            return false;
+    }
+    return ensure_info(b, a.count);
+}
+
+void freeze(BatchCtx &b, bool f)
+{
+    for (int k = 0; k < kMaxSlots; k++) b.slot[k].sc.fixed = f;
+}
+
+// Enqueue instances on b.main: fork, per-stream sequences, join; `last` adds
+// the copy of every instance's status to the host.  Every forked stream is
+// joined back into b.main, also when enqueueing fails half way, so that a
+// capture can always be ended.
 bool batch_enqueue(BatchCtx &b, const BatchArgs &a, const BatchCfg &cfg,
                    const std::vector<std::vector<int>> &lists, bool last)
 {
     if (!ok(hipEventRecord(b.fork, b.main))) return false;
-    for (int k = 0; k < (int)lists.size(); k++) {
+    const int ns = (int)lists.size();
+    int forked = 0;
+    bool good = true;
+    for (; forked < ns; forked++)
+        if (!ok(hipStreamWaitEvent(b.st[forked], b.fork, 0))) {
+            good = false;
+            break;
+        }
+    for (int k = 0; k < forked && good; k++) {
         hipStream_t st = b.st[k];
-        if (!ok(hipStreamWaitEvent(st, b.fork, 0))) return false;
         for (int i : lists[k]) {
-            Run r(b.slot[k], a.m[i], a.n[i], a.s[i], a.X[i], a.ldX[i], st);
+            Run r(b.slot[k], b.slot[k].sc, a.m[i], a.n[i], a.s[i], a.X[i], a.ldX[i], st);
             r.set_wg_scale(cfg.wg_scale);
             if (a.mx) r.set_mixed(a.mx, a.X32[i], a.ldX32[i]);
-            if (!r.prepare() || !r.run(a.variant)) return false;
-            if (!r.copy_R_out(a.R[i], a.ldR[i], hipMemcpyDeviceToDevice)) return false;
-            if (!ok(hipMemcpyAsync(b.info_d + 2 * i, b.slot[k].status_d, 2 * sizeof(int),
-                                   hipMemcpyDeviceToDevice, st)))
-                return false;
+            good = r.prepare() && r.run(a.variant) &&
+                   r.copy_R_out(a.R[i], a.ldR[i], hipMemcpyDeviceToDevice) &&
+                   ok(hipMemcpyAsync(b.info_d + 2 * i, b.slot[k].sc.status_d, 2 * sizeof(int),
+                                     hipMemcpyDeviceToDevice, st));
+            if (!good) break;
         }
-        if (!ok(hipEventRecord(b.join[k], st))) return false;
-        if (!ok(hipStreamWaitEvent(b.main, b.join[k], 0))) return false;
     }
-    if (!last) return true;
-    return ok(hipMemcpyAsync(b.info_h, b.info_d, (size_t)a.count * 2 * sizeof(int),
-                             hipMemcpyDeviceToHost, b.main));
+    for (int k = 0; k < forked; k++) {
+        if (!ok(hipEventRecord(b.join[k], b.st[k]))) good = false;
+        if (!ok(hipStreamWaitEvent(b.main, b.join[k], 0))) good = false;
+    }
+    if (good && last)
+        good = ok(hipMemcpyAsync(b.info_h, b.info_d, (size_t)a.count * 2 * sizeof(int),
+                                 hipMemcpyDeviceToHost, b.main));
+    return good;
 }
 
 // Instance chunks of at most kMaxGraphNodes graph nodes each.
@@ -1157,64 +1235,125 @@ std::vector<std::vector<int>> graph_chunks(const BatchArgs &a)
     return chunks;
 }
 
-void drop_graphs(BatchCtx &b)
+bool capturing(hipStream_t st)
 {
-    for (hipGraphExec_t e : b.execs) (void)hipGraphExecDestroy(e);
-    b.execs.clear();
-    b.key.clear();
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(st, &cs) != hipSuccess) {
+        (void)hipGetLastError();
+        return true;  // unknown state: treat as unusable
+    }
+    return cs != hipStreamCaptureStatusNone;
 }
 
-// Run the batch with cfg.  Graph mode replays the captured graphs when the
-// batch is the one they were captured for, else captures them (falling
-// back to direct launches if capture fails); direct mode launches on the
-// streams.
+// Capture the batch into graphs for later replays; nothing runs.  On failure
+// no graph is kept and every stream has left capture mode (else the batch
+// context is marked broken).
+bool batch_capture(BatchCtx &b, const BatchArgs &a, const BatchCfg &cfg)
+{
+    const auto chunks = graph_chunks(a);
+    bool good = true;
+    freeze(b, true);
+    for (size_t ci = 0; ci < chunks.size() && good; ci++) {
+        const auto lists = assign(a, chunks[ci], cfg.streams);
+        if (!ok(hipStreamBeginCapture(b.main, hipStreamCaptureModeThreadLocal))) {
+            good = false;
+            break;
+        }
+        const bool enq = batch_enqueue(b, a, cfg, lists, ci + 1 == chunks.size());
+        hipGraph_t g = nullptr;
+        const bool ended = ok(hipStreamEndCapture(b.main, &g));
+        good = enq && ended && g;
+        hipGraphExec_t e = nullptr;
+        if (good && ok(hipGraphInstantiate(&e, g, nullptr, nullptr, 0)))
+            b.execs.push_back(e);
+        else
+            good = false;
+        if (g) (void)hipGraphDestroy(g);
+    }
+    freeze(b, false);
+    if (good) return true;
+    drop_graphs(b);
+    bool stuck = capturing(b.main);
+    for (int k = 0; k < kMaxSlots; k++) stuck = capturing(b.st[k]) || stuck;
+    if (stuck) b.broken = true;
+    (void)hipGetLastError();
+    tl_hip_error = 0;
+    return false;
+}
+
+// Run the batch with cfg on the current contents of its inputs.  Graph mode
+// replays the graphs captured for this exact batch; a batch met for the
+// first time is launched directly on the streams (which also makes every
+// kernel's first launch, and its launch attributes, happen outside capture)
+// and then captured, without running, for the next call.
 bool batch_run(BatchCtx &b, const BatchArgs &a, const BatchCfg &cfg)
 {
-    std::vector<int> all(a.count);
-    for (int i = 0; i < a.count; i++) all[i] = i;
-    if (!cfg.graph) {
-        // Direct launches may grow the slots' scratch: captured graphs that
-        // refer to the old buffers are dropped first.
-        drop_graphs(b);
-        if (!batch_reserve(b, a, cfg.streams)) return false;
-        if (!batch_enqueue(b, a, cfg, assign(a, all, cfg.streams), true)) return false;
+    const std::vector<int64_t> key = full_key(a, cfg);
+    if (cfg.graph && !b.execs.empty() && key == b.key) {
+        for (hipGraphExec_t e : b.execs)
+            if (!ok(hipGraphLaunch(e, b.main))) return false;
         return ok(hipStreamSynchronize(b.main));
     }
-    const std::vector<int64_t> key = full_key(a, cfg);
-    if (b.execs.empty() || key != b.key) {
-        drop_graphs(b);
-        if (!batch_reserve(b, a, cfg.streams)) return false;
-        const auto chunks = graph_chunks(a);
-        bool good = true;
-        for (size_t ci = 0; ci < chunks.size() && good; ci++) {
-            const auto lists = assign(a, chunks[ci], cfg.streams);
-            hipGraph_t g = nullptr;
-            good = ok(hipStreamBeginCapture(b.main, hipStreamCaptureModeThreadLocal));
-            if (good) {
-                const bool enq = batch_enqueue(b, a, cfg, lists, ci + 1 == chunks.size());
-                good = ok(hipStreamEndCapture(b.main, &g)) && enq && g;
-            }
-            hipGraphExec_t e = nullptr;
-            if (good && ok(hipGraphInstantiate(&e, g, nullptr, nullptr, 0)))
-                b.execs.push_back(e);
-            else
-                good = false;
-            if (g) (void)hipGraphDestroy(g);
-        }
-        if (!good) {
-            drop_graphs(b);
-            (void)hipGetLastError();
-            tl_hip_error = 0;
-            BatchCfg direct = cfg;
-            direct.graph = false;
-            return batch_run(b, a, direct);
-        }
-        b.key = key;
+    drop_graphs(b);
+    if (!batch_reserve(b, a, cfg.streams)) return false;
+    std::vector<int> all(a.count);
+    for (int i = 0; i < a.count; i++) all[i] = i;
+    freeze(b, true);
+    const bool enq = batch_enqueue(b, a, cfg, assign(a, all, cfg.streams), true);
+    freeze(b, false);
+    const bool done = ok(hipStreamSynchronize(b.main));
+    if (!enq || !done) return false;
+    if (cfg.graph && key != b.nocap) {
+        if (batch_capture(b, a, cfg))
+            b.key = key;
+        else if (b.broken)
+            return false;
+        else
+            b.nocap = key;  // launched directly from now on
     }
-    for (hipGraphExec_t e : b.execs)
-        if (!ok(hipGraphLaunch(e, b.main))) return false;
-    return ok(hipStreamSynchronize(b.main));
+    return true;
 }
+
+// Device copies of the batch's FP64 inputs, so that every timing run of the
+// tuner and the final run start from the caller's X (the FP64 form works in
+// place; the mixed form never writes its FP32 input).
+struct InputBackup {
+    double *buf = nullptr;
+    std::vector<size_t> off;
+    ~InputBackup()
+    {
+        if (buf) (void)hipFree(buf);
+    }
+    bool save(const BatchArgs &a, hipStream_t st)
+    {
+        size_t tot = 0;
+        for (int i = 0; i < a.count; i++) {
+            off.push_back(tot);
+            tot += (size_t)a.m[i] * a.n[i];
+        }
+        if (hipMalloc(&buf, tot * sizeof(double)) != hipSuccess) {
+            (void)hipGetLastError();
+            buf = nullptr;
+            return false;
+        }
+        for (int i = 0; i < a.count; i++)
+            if (!ok(hipMemcpy2DAsync(buf + off[i], a.m[i] * sizeof(double), a.X[i],
+                                     a.ldX[i] * sizeof(double), a.m[i] * sizeof(double), a.n[i],
+                                     hipMemcpyDeviceToDevice, st)))
+                return false;
+        return ok(hipStreamSynchronize(st));
+    }
+    bool restore(const BatchArgs &a, hipStream_t st) const
+    {
+        if (!buf) return true;
+        for (int i = 0; i < a.count; i++)
+            if (!ok(hipMemcpy2DAsync(a.X[i], a.ldX[i] * sizeof(double), buf + off[i],
+                                     a.m[i] * sizeof(double), a.m[i] * sizeof(double), a.n[i],
+                                     hipMemcpyDeviceToDevice, st)))
+                return false;
+        return true;
+    }
+};
 
 BatchCfg batch_cfg(const BatchCtx &b, const BatchArgs &a)
 {
@@ -1243,6 +1382,43 @@ bool batch_args_ok(const BatchArgs &a)
     return a.variant >= 0 && a.variant <= 4;
 }
 
+// Time the tuner's candidates on the batch itself; every run starts from the
+// caller's input (restored from bk).  Returns false on a HIP failure.
+bool batch_tune(BatchCtx &b, const BatchArgs &a, const InputBackup &bk, BatchCfg *best_out)
+{
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (!ok(hipEventCreate(&e0)) || !ok(hipEventCreate(&e1))) return false;
+    float best = 1e30f;
+    BatchCfg bestc = *best_out;
+    bool good = true;
+    // Candidates: (streams, workgroup share) from two streams sharing the
+    // device to eight; each with captured graphs and with direct launches.
+    const std::pair<int, double> cands[] = {{2, 0.5}, {4, 0.5}, {8, 0.25}, {8, 0.5}, {4, 1.0}};
+    for (const bool graph : {true, false}) {
+        for (const auto &cd : cands) {
+            if (cd.first > a.count || !good) continue;
+            BatchCfg cand;
+            cand.streams = cd.first;
+            cand.wg_scale = cd.second;
+            cand.graph = graph;
+            // First run: reserves scratch, launches directly and captures;
+            // the second (a replay in graph mode) is the one timed.
+            good = bk.restore(a, b.main) && batch_run(b, a, cand) && bk.restore(a, b.main) &&
+                   ok(hipEventRecord(e0, b.main)) && batch_run(b, a, cand) &&
+                   ok(hipEventRecord(e1, b.main)) && ok(hipEventSynchronize(e1));
+            float ms = 0.f;
+            if (good && ok(hipEventElapsedTime(&ms, e0, e1)) && ms < best) {
+                best = ms;
+                bestc = cand;
+            }
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *best_out = bestc;
+    return good;
+}
+
 int batched_entry(const BatchArgs &a, int *info, void *stream, int tune, int *cfg_out)
 {
     if (!batch_args_ok(a)) return 2;
@@ -1252,49 +1428,31 @@ int batched_entry(const BatchArgs &a, int *info, void *stream, int tune, int *cf
     std::lock_guard<std::mutex> lk(c.mu);
     if (!ctx_init(c)) return 2;
     BatchCtx &b = g_batch[dev];
-    if (!batch_init(b)) return 2;
+    if (b.broken || !batch_init(b)) return 2;
     // Order after the caller's stream and the library's earlier calls.
     if (!ok(hipStreamSynchronize((hipStream_t)stream))) return 2;
     if (c.done_valid && !ok(hipEventSynchronize(c.done))) return 2;
     BatchCfg cfg = batch_cfg(b, a);
     if (tune) {
-        hipEvent_t e0, e1;
-        if (!ok(hipEventCreate(&e0)) || !ok(hipEventCreate(&e1))) return 2;
-        float best = 1e30f;
-        BatchCfg bestc = cfg;
-        // Candidates: (streams, workgroup share) from one stream owning the
-        // device to eight streams sharing it.
-        const std::pair<int, double> cands[] = {{2, 0.5}, {4, 0.5}, {8, 0.25}, {8, 0.5}, {4, 1.0}};
-        for (const bool graph : {true, false}) {
-            for (const auto &cd : cands) {
-                if (cd.first > a.count) continue;
-                BatchCfg cand;
-                cand.streams = cd.first;
-                cand.wg_scale = cd.second;
-                cand.graph = graph;
-                if (graph && !batch_run(b, a, cand)) return 2;  // capture
-                if (!ok(hipEventRecord(e0, b.main))) return 2;
-                if (!batch_run(b, a, cand)) return 2;
-                if (!ok(hipEventRecord(e1, b.main)) || !ok(hipEventSynchronize(e1))) return 2;
-                float ms = 0.f;
-                (void)hipEventElapsedTime(&ms, e0, e1);
-                if (ms < best) {
-                    best = ms;
-                    bestc = cand;
+        // Tuning needs a device copy of the FP64 inputs (sum of 8 m n bytes);
+        // without room for it the call runs untuned.
+        InputBackup bk;
+        if (a.mx != 0 || bk.save(a, b.main)) {
+            BatchCfg best = cfg;
+            if (!batch_tune(b, a, bk, &best)) return 2;
+            const std::vector<int64_t> sig = shape_sig(a);
+            bool found = false;
+            for (auto &t : b.tuned)
+                if (t.first == sig) {
+                    t.second = best;
+                    found = true;
                 }
-            }
+            if (!found) b.tuned.push_back({sig, best});
+            cfg = best;
+            if (!bk.restore(a, b.main)) return 2;
+        } else if (tl_hip_error) {
+            return 2;
         }
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        const std::vector<int64_t> sig = shape_sig(a);
-        bool found = false;
-        for (auto &t : b.tuned)
-            if (t.first == sig) {
-                t.second = bestc;
-                found = true;
-            }
-        if (!found) b.tuned.push_back({sig, bestc});
-        cfg = bestc;
     }
     if (!batch_run(b, a, cfg)) return 2;
     int worst = 0;
@@ -1319,6 +1477,32 @@ int batched_entry(const BatchArgs &a, int *info, void *stream, int tune, int *cf
 }
 
 }  // namespace
+
+size_t scratch_p_need(int64_t n) { return (size_t)(n + 3 * kMaxS) * (4 * kMaxS) + 64; }
+size_t scratch_r_need(int64_t n) { return (size_t)n * (n + kMaxS) + 64; }
+size_t scratch_tbig_need(int64_t n) { return (size_t)(n + kMaxS) * kMaxS; }
+size_t scratch_cbuf_need(int64_t n) { return (size_t)(n + 2 * kMaxS) * 2 * kMaxS; }
+size_t scratch_stash_need() { return 6 * (size_t)kSlot; }
+
+int run_on_fixed_scratch(int variant, int64_t m, int64_t n, int64_t s, double *dX, int64_t ldX,
+                         double *dR, int64_t ldR, int *dInfo, Scratch &sc, int ncu, void *stream)
+{
+    if (!dX || !dR || !shape_ok(m, n, s, ldX, ldR) || variant < ORTH_P1S || variant > ORTH_P2S)
+        return 2;
+    tl_hip_error = 0;
+    // A context without events or staging: per-kernel timing stays off.
+    DevCtx c;
+    c.ncu = ncu;
+    hipStream_t st = (hipStream_t)stream;
+    Run r(c, sc, m, n, s, dX, ldX, st);
+    if (!r.prepare() || !r.run(variant) || !r.copy_R_out(dR, ldR, hipMemcpyDeviceToDevice))
+        return 2;
+    if (dInfo && !ok(hipMemcpyAsync(dInfo, sc.status_d, 2 * sizeof(int), hipMemcpyDeviceToDevice,
+                                    st)))
+        return 2;
+    return 0;
+}
+
 }  // namespace orth
 
 using namespace orth;
@@ -1391,6 +1575,9 @@ int orth_bcgs_mixed_device(int variant, int precision, int64_t m, int64_t n, int
     std::lock_guard<std::mutex> lk(c.mu);
     if (!ctx_init(c)) return 2;
     if (!fused_shape) {
+        // The widened copy overwrites Q: order it after the library's last
+        // call, which may still be writing there.
+        if (c.done_valid && !ok(hipStreamWaitEvent((hipStream_t)stream, c.done, 0))) return 2;
         if (!ok(launch_widen(dX32, ldX32, dQ, ldQ, m, n, (hipStream_t)stream))) return 2;
         return run_device(c, variant, m, n, s, dQ, ldQ, dR, ldR, hipMemcpyDeviceToDevice,
                           (hipStream_t)stream);
@@ -1471,9 +1658,9 @@ int orth_gram_device(int64_t m, const double *dX, int64_t ldX, int64_t a0, int64
     if (c.done_valid && !ok(hipStreamWaitEvent(st, c.done, 0))) return 2;
     ProjPlan pl = plan_proj(m, na, nb, kProjTargetWG);
     if (pl.lds_bytes > 64 * 1024) return 2;
-    if (!ensure(c.part, c.part_n, (size_t)pl.nwg * na * nb)) return 2;
-    if (!ok(launch_proj(pl, dX, ldX, m, a0, na, b0, nb, c.part, st))) return 2;
-    if (!ok(launch_reduce(c.part, pl.nwg, na, nb, dC, ldc, st))) return 2;
+    if (!grow(c.sc, c.sc.part, c.sc.part_n, (size_t)pl.nwg * na * nb)) return 2;
+    if (!ok(launch_proj(pl, dX, ldX, m, a0, na, b0, nb, c.sc.part, st))) return 2;
+    if (!ok(launch_reduce(c.sc.part, pl.nwg, na, nb, dC, ldc, st))) return 2;
     if (!ok(hipEventRecord(c.done, st))) return 2;
     c.done_valid = true;
     return 0;
@@ -1581,33 +1768,46 @@ int orth_update_device(int64_t m, double *dX, int64_t ldX, int64_t col0, int64_t
     hipStream_t st = (hipStream_t)stream;
     if (ctx.done_valid && !ok(hipStreamWaitEvent(st, ctx.done, 0))) return 2;
     // Coefficients are staged with S zero-padded columns and the factor with
-    // leading dimension kLdT, the layout the update kernel reads.
-    double *Wp = nullptr, *Tp = nullptr;
+    // leading dimension kLdT, the layout the update kernel reads.  The
+    // staging buffers are released on every return (the call is synchronous).
+    struct Staging {
+        double *p = nullptr;
+        ~Staging()
+        {
+            if (p) (void)hipFree(p);
+        }
+    } Wp, Tp;
     if (c > 0) {
-        if (!ok(hipMalloc(&Wp, (size_t)c * S * sizeof(double)))) return 2;
-        if (!ok(launch_fill(Wp, (size_t)c * S, st)) ||
-            !ok(hipMemcpy2DAsync(Wp, c * sizeof(double), dW, ldW * sizeof(double),
-                                 c * sizeof(double), w, hipMemcpyDeviceToDevice, st)))
+        if (!ok(hipMalloc(&Wp.p, (size_t)c * S * sizeof(double)))) return 2;
+        if (!ok(launch_fill(Wp.p, (size_t)c * S, st)) ||
+            !ok(hipMemcpy2DAsync(Wp.p, c * sizeof(double), dW, ldW * sizeof(double),
+                                 c * sizeof(double), w, hipMemcpyDeviceToDevice, st))) {
+            (void)hipStreamSynchronize(st);
             return 2;
+        }
     }
     if (dT) {
-        if (!ok(hipMalloc(&Tp, (size_t)kLdT * kLdT * sizeof(double)))) return 2;
-        if (!ok(launch_fill(Tp, (size_t)kLdT * kLdT, st)) ||
-            !ok(hipMemcpy2DAsync(Tp, kLdT * sizeof(double), dT, w * sizeof(double),
-                                 w * sizeof(double), w, hipMemcpyDeviceToDevice, st)))
+        if (!ok(hipMalloc(&Tp.p, (size_t)kLdT * kLdT * sizeof(double)))) {
+            (void)hipStreamSynchronize(st);
             return 2;
+        }
+        if (!ok(launch_fill(Tp.p, (size_t)kLdT * kLdT, st)) ||
+            !ok(hipMemcpy2DAsync(Tp.p, kLdT * sizeof(double), dT, w * sizeof(double),
+                                 w * sizeof(double), w, hipMemcpyDeviceToDevice, st))) {
+            (void)hipStreamSynchronize(st);
+            return 2;
+        }
     }
     Upd u;
     u.col0 = col0;
     u.w = (int)w;
     u.c = (int)c;
-    u.W = Wp;
+    u.W = Wp.p;
     u.ldW = c;
-    u.T = Tp;
-    bool good = ok(launch_update(S, dX, ldX, m, &u, 1, st)) && ok(hipStreamSynchronize(st));
-    if (Wp) (void)hipFree(Wp);
-    if (Tp) (void)hipFree(Tp);
-    return good ? 0 : 2;
+    u.T = Tp.p;
+    const bool launched = ok(launch_update(S, dX, ldX, m, &u, 1, st));
+    const bool done = ok(hipStreamSynchronize(st));
+    return launched && done ? 0 : 2;
 }
 
 int orth_sync(void)
@@ -1625,12 +1825,13 @@ int orth_release(void)
     DevCtx &c = g_ctx[dev];
     std::lock_guard<std::mutex> lk(c.mu);
     if (c.done_valid) (void)hipEventSynchronize(c.done);
-    double **bufs[] = {&c.part, &c.P[0], &c.P[1], &c.Rd, &c.Xd};
+    double **bufs[] = {&c.sc.part, &c.sc.P[0], &c.sc.P[1], &c.sc.Rd, &c.sc.Tbig, &c.sc.Cbuf,
+                       &c.Xd};
     for (double **b : bufs) {
         if (*b) (void)hipFree(*b);
         *b = nullptr;
     }
-    c.part_n = c.P_n = c.Rd_n = c.Xd_n = 0;
+    c.sc.part_n = c.sc.P_n = c.sc.Rd_n = c.sc.Tbig_n = c.sc.Cbuf_n = c.Xd_n = 0;
     return 0;
 }
 

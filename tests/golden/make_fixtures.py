@@ -2,10 +2,11 @@
 
 The reference ships no expected outputs (its test driver only prints), so
 the expected factors here come from independent computations:
-  * test_correct.npz: the reference driver's own input (TESTS/test_orth.c:
-    42-44, glibc rand() after srand(0), m = 550, n = 35..39 share the first
-    columns) and LAPACK Householder R factors (numpy.linalg.qr) with a
-    non-negative diagonal;
+  * test_correct.npz: the reference driver's own inputs X35..X39
+    (TESTS/test_orth.c:42-44 with the calls of :284-288: glibc rand() after
+    srand(0), X[i] = rand() % (10 m n) / (10 m n + 1) for m = 550 and each
+    n = 35..39) and their LAPACK Householder R factors R35..R39
+    (numpy.linalg.qr) with a non-negative diagonal;
   * closed_form.npz: X = Q0 R0 with known orthonormal Q0 and upper R0, so
     the exact factors are (Q0, R0);
   * brute_force.npz: a 10 x 6 matrix whose R factor is computed by classical
@@ -53,9 +54,12 @@ def exact_r(X, digits=60):
 
 
 def main():
-    X = test_correct_matrix(550, 39, rank=0)
-    Rs = {"R%d" % n: house_r(X[:, :n]) for n in range(35, 40)}
-    np.savez_compressed(os.path.join(HERE, "test_correct.npz"), X=X, **Rs)
+    arrays = {}
+    for n in range(35, 40):
+        X = test_correct_matrix(550, n, rank=0)
+        arrays["X%d" % n] = X
+        arrays["R%d" % n] = house_r(X)
+    np.savez_compressed(os.path.join(HERE, "test_correct.npz"), **arrays)
 
     rng = np.random.default_rng(7)
     m, n = 300, 24

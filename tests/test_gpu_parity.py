@@ -60,7 +60,7 @@ def check_against_oracle(variant, X, s, rtol=1e-11):
 @pytest.mark.parametrize("n", [35, 36, 37, 38, 39])
 def test_reference_driver_input(variant, n):
     g = np.load(os.path.join(GOLDEN, "test_correct.npz"))
-    X = np.asfortranarray(g["X"][:, :n])
+    X = np.asfortranarray(g["X%d" % n])
     Qg, Rg, _ = check_against_oracle(variant, X, 5)
     Rref = g["R%d" % n]
     assert np.abs(Rg - Rref).max() <= 1e-11 * np.abs(Rref).max()
