@@ -104,6 +104,8 @@ bool ctx_init(DevCtx &c)
     if (!ok(hipEventCreateWithFlags(&c.done, hipEventDisableTiming))) return false;
     if (!ok(hipMalloc(&c.sc.stash, scratch_stash_need() * sizeof(double)))) return false;
     if (!ok(hipMalloc(&c.sc.status_d, 4 * sizeof(int)))) return false;
+    if (!ok(hipMalloc(&c.sc.tickets, kTailTickets * sizeof(int)))) return false;
+    c.sc.tickets_n = kTailTickets;
     if (!ok(hipHostMalloc(&c.status_h, 4 * sizeof(int), hipHostMallocDefault))) return false;
     int dev = 0;
     (void)hipGetDevice(&dev);
@@ -119,7 +121,8 @@ int64_t env_bits()
 {
     static const char *const names[] = {"ORTH_UNFUSED", "ORTH_UPDATE_ROWWISE", "ORTH_PASS256",
                                         "ORTH_GENERIC_PASS", "ORTH_MIXED_PASS32",
-                                        "ORTH_PASS_TILE32", "ORTH_PASS_LOAD8"};
+                                        "ORTH_PASS_TILE32", "ORTH_PASS_LOAD8",
+                                        "ORTH_SEPARATE_REDUCE"};
     int64_t b = 0;
     for (int i = 0; i < (int)(sizeof(names) / sizeof(names[0])); i++)
         if (getenv(names[i])) b |= (int64_t)1 << i;
@@ -206,14 +209,16 @@ class Run {
         if (!ok(launch_fill(sc_.Rd, rmax, st_))) return false;
         if (!ok(launch_fill(sc_.stash, scratch_stash_need(), st_))) return false;
         if (!ok(launch_fill_int(sc_.status_d, 4, st_))) return false;
+        if (sc_.tickets && !ok(launch_fill_int(sc_.tickets, sc_.tickets_n, st_))) return false;
         Rd_ = sc_.Rd;
         return true;
     }
 
     int64_t bw(int64_t b) const { return b == 0 ? s_ : std::min<int64_t>(s_, n_ - b * s_); }
 
-    // P <- [X(:, a0:a0+na)]^T [X(:, b0:b0+nb)]; returns the buffer (ld na).
-    double *proj(int64_t a0, int64_t na, int64_t b0, int64_t nb)
+    // P <- [X(:, a0:a0+na)]^T [X(:, b0:b0+nb)], then the small step `after`
+    // on it when given; returns the buffer (ld na).
+    double *proj(int64_t a0, int64_t na, int64_t b0, int64_t nb, const SmallArgs *after = nullptr)
     {
         ProjPlan pl = plan_proj(m_, na, nb, std::max(1, (int)(kProjTargetWG * wg_scale_)));
         if (sc_.fixed && (size_t)pl.nwg * na * nb > sc_.part_n) {
@@ -235,16 +240,14 @@ class Run {
             return nullptr;
         pcur_ ^= 1;
         double *P = sc_.P[pcur_];
-        if (!timed(c_, ORTH_K_REDUCE, st_, (double)pl.nwg * na * nb,
-                   8.0 * ((double)pl.nwg + 1) * na * nb,
-                   [&] { return launch_reduce(sc_.part, pl.nwg, na, nb, P, na, st_); }))
-            return nullptr;
+        if (!sync(pl.nwg, na, nb, P, after)) return nullptr;
         passes_++;
         syncs_++;
         return P;
     }
 
-    bool small(SmallArgs a)
+    // The library's pointers and constants of a small step.
+    void finish_small(SmallArgs &a) const
     {
         a.m = m_;
         a.R = Rd_;
@@ -261,6 +264,11 @@ class Run {
             a.n1 = n1_;
             a.S = S_;
         }
+    }
+
+    bool small(SmallArgs a)
+    {
+        finish_small(a);
         if (!timed(c_, ORTH_K_SMALL, st_, 0.0, 0.0, [&] { return launch_small(a, st_); }))
             return false;
         if (debug_) dump(a);
@@ -324,12 +332,10 @@ class Run {
     {
         const int ops[3] = {OP_CQR1, OP_CQR2, OP_CQR3};
         for (int k = 0; k < 3; k++) {
-            double *P = proj(col0, w, col0, w);
-            if (!P) return false;
-            SmallArgs a = sa(ops[k], 0, blk, 0, w, 0, P, w);
+            SmallArgs a = sa(ops[k], 0, blk, 0, w, 0, nullptr, w);
             a.Rdst = Rdst;
             a.ldRdst = ldRdst;
-            if (!small(a)) return false;
+            if (!proj(col0, w, col0, w, &a)) return false;
             if (!update(upd(col0, w, 0, nullptr, 0, Cq_ + k * kSlot))) return false;
         }
         return true;
@@ -345,8 +351,9 @@ class Run {
         const int64_t w1 = bw(1);
         {
             // [Q1 X2]^T X2 (:111-114); Sigma, R12 = S (:115-119).
-            double *P = proj(0, s_ + w1, s_, w1);
-            if (!P || !small(sa(OP_OPEN_PYTH, 0, 1, s_, w1, 0, P, s_ + w1))) return false;
+            SmallArgs q = sa(OP_OPEN_PYTH, 0, 1, s_, w1, 0, nullptr, s_ + w1);
+            double *P = proj(0, s_ + w1, s_, w1, &q);
+            if (!P) return false;
             // X2 = (X2 - Q1 S) Sigma^{-1} (:121-124).
             if (!update(upd(s_, w1, s_, P, s_ + w1, Sig_))) return false;
         }
@@ -355,8 +362,9 @@ class Run {
             const int64_t wb = bw(b), wn = (b + 1 < N_) ? bw(b + 1) : 0;
             const int64_t na = c + wb + wn;
             // The one sync: [Q Q^_b]^T [Q^_b X_n] and X_n^T X_n (:127-133, :182-197).
-            double *P = proj(0, na, c, wb + wn);
-            if (!P || !small(sa(OP_CLOSE_OPEN, F_PYTH_NEXT, b, c, wb, wn, P, na))) return false;
+            SmallArgs q = sa(OP_CLOSE_OPEN, F_PYTH_NEXT, b, c, wb, wn, nullptr, na);
+            double *P = proj(0, na, c, wb + wn, &q);
+            if (!P) return false;
             // Q_b = (Q^_b - Q Y) Omega^{-1} (:139-142, :206-209) and the next
             // block's first pass (:173-176) in one pass.
             Upd u0 = upd(c, wb, c, P, na, Om_);
@@ -379,8 +387,9 @@ class Run {
         const int64_t w1 = bw(1);
         {
             // S = Q1^T X2 (:275-278); V2 = X2 - Q1 S (:280-281).
-            double *P = proj(0, s_, s_, w1);
-            if (!P || !small(sa(OP_PLAIN_S, 0, 1, s_, w1, 0, P, s_))) return false;
+            SmallArgs q = sa(OP_PLAIN_S, 0, 1, s_, w1, 0, nullptr, s_);
+            double *P = proj(0, s_, s_, w1, &q);
+            if (!P) return false;
             if (!update(upd(s_, w1, s_, P, s_, nullptr))) return false;
         }
         int64_t c = s_;
@@ -388,8 +397,9 @@ class Run {
             const int64_t wb = bw(b), wn = (b + 1 < N_) ? bw(b + 1) : 0;
             const int64_t na = c + wb;
             // [Q V_b]^T [V_b X_n] (:284-287, :330-342).
-            double *P = proj(0, na, c, wb + wn);
-            if (!P || !small(sa(OP_CLOSE_OPEN, F_1S, b, c, wb, wn, P, na))) return false;
+            SmallArgs q = sa(OP_CLOSE_OPEN, F_1S, b, c, wb, wn, nullptr, na);
+            double *P = proj(0, na, c, wb + wn, &q);
+            if (!P) return false;
             Upd u0 = upd(c, wb, c, P, na, Om_);
             if (wn > 0) {
                 Upd u1 = upd(c + wb, wn, c + wb, P + wb * na, na, nullptr);
@@ -409,11 +419,13 @@ class Run {
         int64_t c = s_;
         for (int64_t b = 1; b < N_; b++) {
             const int64_t wb = bw(b), na = c + wb;
-            double *P = proj(0, na, c, wb);  // :421-424
-            if (!P || !small(sa(OP_PIPI1, 0, b, c, wb, 0, P, na))) return false;
+            SmallArgs q = sa(OP_PIPI1, 0, b, c, wb, 0, nullptr, na);  // :421-424
+            double *P = proj(0, na, c, wb, &q);
+            if (!P) return false;
             if (!update(upd(c, wb, c, P, na, Sig_))) return false;  // :431-434
-            P = proj(0, na, c, wb);  // :435-438
-            if (!P || !small(sa(OP_PIPI2, 0, b, c, wb, 0, P, na))) return false;
+            q = sa(OP_PIPI2, 0, b, c, wb, 0, nullptr, na);  // :435-438
+            P = proj(0, na, c, wb, &q);
+            if (!P) return false;
             if (!update(upd(c, wb, c, P, na, Om_))) return false;  // :446-449
             c += wb;
         }
@@ -427,12 +439,14 @@ class Run {
         int64_t c = s_;
         for (int64_t b = 1; b < N_; b++) {
             const int64_t wb = bw(b);
-            double *P = proj(0, c, c, wb);  // S = Q^T X (:512-515)
-            if (!P || !small(sa(OP_PLAIN_S, 0, b, c, wb, 0, P, c))) return false;
+            SmallArgs q = sa(OP_PLAIN_S, 0, b, c, wb, 0, nullptr, c);  // S = Q^T X (:512-515)
+            double *P = proj(0, c, c, wb, &q);
+            if (!P) return false;
             if (!update(upd(c, wb, c, P, c, nullptr))) return false;  // :518
             if (!intra_qr(b, c, wb, Sig_, kLdT)) return false;         // :520-523
-            P = proj(0, c, c, wb);  // W = Q^T Q^ (:524-527)
-            if (!P || !small(sa(OP_I2_W, 0, b, c, wb, 0, P, c))) return false;
+            q = sa(OP_I2_W, 0, b, c, wb, 0, nullptr, c);  // W = Q^T Q^ (:524-527)
+            P = proj(0, c, c, wb, &q);
+            if (!P) return false;
             if (!update(upd(c, wb, c, P, c, nullptr))) return false;  // :530
             if (!intra_qr(b, c, wb, Om_, kLdT)) return false;          // :532-535
             if (!small(sa(OP_RBB, 0, b, c, wb, 0, nullptr, 1))) return false;  // :537-541
@@ -449,8 +463,9 @@ class Run {
         const int64_t w1 = bw(1);
         {
             // S = Q1^T X2, R12 = S (:605-609); V = X2 - Q1 S (:610-611); TSQR (:612-614).
-            double *P = proj(0, s_, s_, w1);
-            if (!P || !small(sa(OP_PLAIN_S, 0, 1, s_, w1, 0, P, s_))) return false;
+            SmallArgs q = sa(OP_PLAIN_S, 0, 1, s_, w1, 0, nullptr, s_);
+            double *P = proj(0, s_, s_, w1, &q);
+            if (!P) return false;
             if (!update(upd(s_, w1, s_, P, s_, nullptr))) return false;
             if (!intra_qr(1, s_, w1, Sig_, kLdT)) return false;
         }
@@ -460,8 +475,9 @@ class Run {
             const int64_t na = c + wb;
             // [Q Q^_b]^T [Q^_b X_n] (:617-620, :658-675); the unused
             // X_n^T X_n Gram of :660-662 is not formed.
-            double *P = proj(0, na, c, wb + wn);
-            if (!P || !small(sa(OP_CLOSE_OPEN, 0, b, c, wb, wn, P, na))) return false;
+            SmallArgs q = sa(OP_CLOSE_OPEN, 0, b, c, wb, wn, nullptr, na);
+            double *P = proj(0, na, c, wb + wn, &q);
+            if (!P) return false;
             Upd u0 = upd(c, wb, c, P, na, Om_);
             if (wn > 0) {
                 Upd u1 = upd(c + wb, wn, c + wb, P + wb * na, na, nullptr);
@@ -513,8 +529,9 @@ class Run {
     // runs several calls side by side).
     void set_wg_scale(double f) { wg_scale_ = std::min(1.0, std::max(f, 1e-3)); }
 
-    // One fused pass: the pending update, then P <- A^T B (na == 0: none).
-    double *pass(int64_t a0, int64_t na, int64_t b0, int64_t nb)
+    // One fused pass: the pending update, then P <- A^T B (na == 0: none),
+    // then the small step `after` on P when given.
+    double *pass(int64_t a0, int64_t na, int64_t b0, int64_t nb, const SmallArgs *after = nullptr)
     {
         // The tile starts at the first column any part of the pass touches
         // (column 0 whenever a prefix product is applied).
@@ -624,11 +641,40 @@ class Run {
         if (na == 0) return sc_.P[pcur_];
         pcur_ ^= 1;
         double *P = sc_.P[pcur_];
-        if (!timed(c_, ORTH_K_REDUCE, st_, (double)nwg * na * nb, 8.0 * ((double)nwg + 1) * na * nb,
-                   [&] { return launch_reduce(sc_.part, nwg, na, nb, P, na, st_); }))
-            return nullptr;
         syncs_++;
-        return P;
+        return sync(nwg, na, nb, P, after) ? P : nullptr;
+    }
+
+    // The synchronization after a projection: P <- the fixed-order sum of the
+    // nwg partials, then the small step `after` (if any) on P -- in one
+    // launch when the scratch has the ticket counter, else as two.
+    bool sync(int nwg, int64_t na, int64_t nb, double *P, const SmallArgs *after)
+    {
+        const double rflops = (double)nwg * na * nb, rbytes = 8.0 * ((double)nwg + 1) * na * nb;
+        if (after && sc_.tickets && !getenv("ORTH_SEPARATE_REDUCE")) {
+            SmallArgs a = *after;
+            a.P = P;
+            finish_small(a);
+            if (!timed(c_, ORTH_K_REDUCE, st_, rflops, rbytes, [&] {
+                    return launch_reduce_small(sc_.part, nwg, na, nb, P, sc_.tickets, a, st_);
+                }))
+                return false;
+            if (debug_) dump(a);
+            return true;
+        }
+        if (!timed(c_, ORTH_K_REDUCE, st_, rflops, rbytes,
+                   [&] { return launch_reduce(sc_.part, nwg, na, nb, P, na, st_); }))
+            return false;
+        if (!after) return true;
+        SmallArgs a = *after;
+        a.P = P;
+        return small(a);
+    }
+
+    // A pass followed by the small step s on its reduced projection.
+    bool pass_small(int64_t a0, int64_t na, int64_t b0, int64_t nb, const SmallArgs &s)
+    {
+        return pass(a0, na, b0, nb, &s) != nullptr;
     }
 
     bool flush()
@@ -662,12 +708,10 @@ class Run {
     {
         const int ops[3] = {OP_CQR1, OP_CQR2, OP_CQR3};
         for (int k = 0; k < 3; k++) {
-            double *P = pass(col0, w, col0, w);
-            if (!P) return false;
-            SmallArgs a = sa(ops[k], 0, blk, 0, w, 0, P, w);
+            SmallArgs a = sa(ops[k], 0, blk, 0, w, 0, nullptr, w);
             a.Rdst = Rdst;
             a.ldRdst = ldRdst;
-            if (!small(a)) return false;
+            if (!pass(col0, w, col0, w, &a)) return false;
             pend_group(col0, w, 0, 0, Cq_ + k * kSlot);
         }
         return true;
@@ -689,8 +733,7 @@ class Run {
         // Mixed form: the projection of a block's raw columns (first pass)
         // and its first-pass update run in the reduced precision.
         if (lp_first_pass()) lp_from_ = s_;
-        double *P = pass(0, s_ + w1, s_, w1);
-        if (!P || !small(sa(OP_OPEN_PYTH, 0, 1, s_, w1, 0, P, s_ + w1))) return false;
+        if (!pass_small(0, s_ + w1, s_, w1, sa(OP_OPEN_PYTH, 0, 1, s_, w1, 0, nullptr, s_ + w1))) return false;
         pend_.k1 = (int)s_;
         pend_group(s_, w1, 0, s_, Sig_, lp_first_pass());
         int64_t c = s_;
@@ -698,8 +741,7 @@ class Run {
             const int64_t wb = bw(b), wn = (b + 1 < N_) ? bw(b + 1) : 0;
             const int64_t na = c + wb + wn;
             if (lp_first_pass() && wn > 0) lp_from_ = c + wb;
-            P = pass(0, na, c, wb + wn);
-            if (!P || !small(sa(OP_CLOSE_OPEN, F_PYTH_NEXT, b, c, wb, wn, P, na))) return false;
+            if (!pass_small(0, na, c, wb + wn, sa(OP_CLOSE_OPEN, F_PYTH_NEXT, b, c, wb, wn, nullptr, na))) return false;
             pend_close_open(c, wb, wn, true, false);
             c += wb;
         }
@@ -711,16 +753,14 @@ class Run {
         if (!intra_qr_f(0, 0, s_, Rd_, ldR_)) return false;
         if (N_ == 1) return flush();
         const int64_t w1 = bw(1);
-        double *P = pass(0, s_, s_, w1);
-        if (!P || !small(sa(OP_PLAIN_S, 0, 1, s_, w1, 0, P, s_))) return false;
+        if (!pass_small(0, s_, s_, w1, sa(OP_PLAIN_S, 0, 1, s_, w1, 0, nullptr, s_))) return false;
         pend_.k1 = (int)s_;
         pend_group(s_, w1, 0, s_, nullptr);
         int64_t c = s_;
         for (int64_t b = 1; b < N_; b++) {
             const int64_t wb = bw(b), wn = (b + 1 < N_) ? bw(b + 1) : 0;
             const int64_t na = c + wb;
-            P = pass(0, na, c, wb + wn);
-            if (!P || !small(sa(OP_CLOSE_OPEN, F_1S | F_SEQ, b, c, wb, wn, P, na))) return false;
+            if (!pass_small(0, na, c, wb + wn, sa(OP_CLOSE_OPEN, F_1S | F_SEQ, b, c, wb, wn, nullptr, na))) return false;
             pend_close_open(c, wb, wn, false, true);
             c += wb;
         }
@@ -733,12 +773,10 @@ class Run {
         int64_t c = s_;
         for (int64_t b = 1; b < N_; b++) {
             const int64_t wb = bw(b), na = c + wb;
-            double *P = pass(0, na, c, wb);
-            if (!P || !small(sa(OP_PIPI1, 0, b, c, wb, 0, P, na))) return false;
+            if (!pass_small(0, na, c, wb, sa(OP_PIPI1, 0, b, c, wb, 0, nullptr, na))) return false;
             pend_.k1 = (int)c;
             pend_group(c, wb, 0, c, Sig_);
-            P = pass(0, na, c, wb);
-            if (!P || !small(sa(OP_PIPI2, 0, b, c, wb, 0, P, na))) return false;
+            if (!pass_small(0, na, c, wb, sa(OP_PIPI2, 0, b, c, wb, 0, nullptr, na))) return false;
             pend_.k1 = (int)c;
             pend_group(c, wb, 0, c, Om_);
             c += wb;
@@ -752,13 +790,11 @@ class Run {
         int64_t c = s_;
         for (int64_t b = 1; b < N_; b++) {
             const int64_t wb = bw(b);
-            double *P = pass(0, c, c, wb);
-            if (!P || !small(sa(OP_PLAIN_S, 0, b, c, wb, 0, P, c))) return false;
+            if (!pass_small(0, c, c, wb, sa(OP_PLAIN_S, 0, b, c, wb, 0, nullptr, c))) return false;
             pend_.k1 = (int)c;
             pend_group(c, wb, 0, c, nullptr);
             if (!intra_qr_f(b, c, wb, Sig_, kLdT)) return false;
-            P = pass(0, c, c, wb);
-            if (!P || !small(sa(OP_I2_W, 0, b, c, wb, 0, P, c))) return false;
+            if (!pass_small(0, c, c, wb, sa(OP_I2_W, 0, b, c, wb, 0, nullptr, c))) return false;
             pend_.k1 = (int)c;
             pend_group(c, wb, 0, c, nullptr);
             if (!intra_qr_f(b, c, wb, Om_, kLdT)) return false;
@@ -773,8 +809,7 @@ class Run {
         if (!intra_qr_f(0, 0, s_, Rd_, ldR_)) return false;
         if (N_ == 1) return flush();
         const int64_t w1 = bw(1);
-        double *P = pass(0, s_, s_, w1);
-        if (!P || !small(sa(OP_PLAIN_S, 0, 1, s_, w1, 0, P, s_))) return false;
+        if (!pass_small(0, s_, s_, w1, sa(OP_PLAIN_S, 0, 1, s_, w1, 0, nullptr, s_))) return false;
         pend_.k1 = (int)s_;
         pend_group(s_, w1, 0, s_, nullptr);
         if (!intra_qr_f(1, s_, w1, Sig_, kLdT)) return false;
@@ -782,8 +817,7 @@ class Run {
         for (int64_t b = 1; b < N_; b++) {
             const int64_t wb = bw(b), wn = (b + 1 < N_) ? bw(b + 1) : 0;
             const int64_t na = c + wb;
-            P = pass(0, na, c, wb + wn);
-            if (!P || !small(sa(OP_CLOSE_OPEN, 0, b, c, wb, wn, P, na))) return false;
+            if (!pass_small(0, na, c, wb + wn, sa(OP_CLOSE_OPEN, 0, b, c, wb, wn, nullptr, na))) return false;
             pend_close_open(c, wb, wn, false, false);
             if (wn > 0 && !intra_qr_f(b + 1, c + wb, wn, Sig_, kLdT)) return false;
             c += wb;

@@ -104,6 +104,15 @@ hipError_t launch_reduce(const double *part, int nwg, int64_t na, int64_t nb,
 
 hipError_t launch_small(const SmallArgs &a, hipStream_t st);
 
+// The synchronization of a pass in one launch (the reference's
+// MPI_Allreduce, SRC/orth.c:114, :133, :188, :197, and the small step after
+// it): workgroups sum slices of the partials in the fixed order of
+// launch_reduce, then take a ticket; the last one to arrive runs the small
+// step `sm` (sm.P = out) and sets the ticket counter back to zero.
+// ticket: one int, zero on entry.
+hipError_t launch_reduce_small(const double *part, int nwg, int64_t na, int64_t nb, double *out,
+                               int *ticket, const SmallArgs &sm, hipStream_t st);
+
 // ---------------------------------------------------------------------------
 // Fused streaming pass: one sweep over the row slab that applies the
 // row-local block updates and accumulates the next projection.

@@ -27,8 +27,13 @@ struct Scratch {
     double *Rd = nullptr;
     size_t Rd_n = 0;
     int *status_d = nullptr;  // 4 ints
+    int *tickets = nullptr;   // ticket counter of the synchronization kernel (zero between launches)
+    size_t tickets_n = 0;
     bool fixed = false;
 };
+
+// Ints reserved for the ticket counter (one used; padded to 256 bytes).
+constexpr size_t kTailTickets = 64;
 
 // Doubles of each buffer a call with n columns needs (any block width).
 size_t scratch_p_need(int64_t n);

@@ -253,6 +253,20 @@ def test_alternate_pass_kernels(variant, env, monkeypatch):
     check_against_oracle(variant, X, 16)
 
 
+@pytest.mark.parametrize("shape", [(6000, 80, 16), (5000, 70, 8), (3000, 96, 32), (70000, 256, 16)])
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_separate_reduction_matches_in_kernel(variant, shape, monkeypatch):
+    """The passes' in-kernel reduction tree (with the small step run by its
+    last workgroup) against separate reduction and small-step launches:
+    both agree with the oracle, and with each other to rounding."""
+    m, n, s = shape
+    X = I.svd_matrix(m, n, 1e4, seed=33)
+    Qa, Ra, _ = check_against_oracle(variant, X, s)
+    monkeypatch.setenv("ORTH_SEPARATE_REDUCE", "1")
+    Qb, Rb, _ = check_against_oracle(variant, X, s)
+    assert np.abs(Ra - Rb).max() <= 1e-12 * np.abs(Ra).max()
+
+
 @pytest.mark.parametrize("variant", VARIANTS)
 @pytest.mark.parametrize("m", [65538, 9906, 96])
 def test_tile_boundary_shapes(variant, m):
