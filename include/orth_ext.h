@@ -55,6 +55,25 @@ int orth_last_status(orth_status_t *st);
 int orth_bcgs_device(int variant, int64_t m, int64_t n, int64_t s, double *dX,
                      int64_t ldX, double *dR, int64_t ldR, void *stream);
 
+/* Allocation-free form of orth_bcgs_device.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * It calls no hipMalloc / hipFree and creates no event: its scratch is a
+ * fixed 48 MiB region of device memory inside the library (one per device,
+ * held until the process ends), whose address is looked up on the first call
+ * on each device.  Limits: n <= 512 columns (larger n returns 2 with a
+ * message on stderr); the other shape rules are orth_bcgs_device's, and the
+ * passes use fewer row chunks when their partial sums would not fit.  R is
+ * written as by orth_bcgs_device; the first breakdown (1-based block, pivot)
+ * goes to dInfo (device memory, 2 ints, may be NULL) instead of
+ * orth_last_status().  Every call on a device shares that device's region:
+ * calls that overlap on one device (other streams, other threads) race.
+ * Enqueues on `stream` and returns; 0, or 2 for an invalid or too large
+ * shape or a HIP error. */
+int orth_bcgs_device_noalloc(int variant, int64_t m, int64_t n, int64_t s, double *dX,
+                             int64_t ldX, double *dR, int64_t ldR, int *dInfo,
+                             void *stream);
+
 /* Mixed-precision form of BCGSI+P-1S (no counterpart in the reference):
  * the input X32 (m x n FP32, ld ldX32, device, not modified) is
  * orthogonalized into Q (m x n FP64, ld ldQ, device) and R (FP64, as in

@@ -97,6 +97,10 @@ def lib():
                                                  _i64p, _i64p, _i64p, _vpp, _i64p, _vpp,
                                                  _i64p, _vpp, _i64p, _ip, ctypes.c_int, _ip,
                                                  _vp]
+    if hasattr(L, "orth_bcgs_device_noalloc"):
+        L.orth_bcgs_device_noalloc.restype = ctypes.c_int
+        L.orth_bcgs_device_noalloc.argtypes = [ctypes.c_int, _i64, _i64, _i64, _vp, _i64, _vp,
+                                               _i64, _vp, _vp]
     _lib = L
     return L
 
@@ -147,6 +151,14 @@ def bcgs_device(variant, m, n, s, x_ptr, ldx, r_ptr, ldr, stream=0):
                                 ldr, stream or None)
     if rc != 0:
         raise RuntimeError("orth_bcgs_device returned %d" % rc)
+
+
+def bcgs_device_noalloc(variant, m, n, s, x_ptr, ldx, r_ptr, ldr, info_ptr=None, stream=0):
+    """Enqueue a variant on device pointers using only the library's fixed
+    device scratch (no allocation); info_ptr (device, 2 int32) receives the
+    first breakdown (block, pivot).  Returns the entry point's code."""
+    return lib().orth_bcgs_device_noalloc(VARIANTS[variant], m, n, s, x_ptr, ldx, r_ptr, ldr,
+                                          info_ptr, stream or None)
 
 
 def bcgs_mixed_device(variant, precision, m, n, s, x32_ptr, ldx, q_ptr, ldq,
