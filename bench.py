@@ -379,6 +379,43 @@ def batched(args):
     # The batched mixed form is switched off (the entry point returns 2).
 
 
+def mixed_only(args):
+    """Only the mixed form at one first-pass precision (for profiling its
+    pass kernel): `--steps` timed factorizations of the FP32-rounded input,
+    one JSON object with the mean ms, LOO and residual against the FP64
+    input."""
+    import torch
+
+    import bcgs_hip as B
+
+    B.lib()
+    m, n, s = args.m, args.n, args.s
+    X0 = make_input(torch, m, n, args.kappa, seed=1)
+    X32 = X0.float()
+    Q = torch.empty_like(X0)
+    R = torch.zeros(n * n, dtype=torch.float64, device="cuda")
+    stream = torch.cuda.current_stream()
+    ts = []
+    for k in range(args.warmup + args.steps):
+        R.zero_()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        B.bcgs_mixed_device("bcgsi2P1s", args.mixed_only, m, n, s, X32.data_ptr(), m,
+                            Q.data_ptr(), m, R.data_ptr(), n, stream.cuda_stream)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        if k >= args.warmup:
+            ts.append(e0.elapsed_time(e1))
+    st = B.status()
+    loo, res, xn = B.accuracy_device(m, n, X0.data_ptr(), m, Q.data_ptr(), m, R.data_ptr(), n,
+                                     stream.cuda_stream)
+    print(json.dumps(finite({"config": "mixed-only", "precision": args.mixed_only, "m": m,
+                             "n": n, "s": s, "kappa": args.kappa,
+                             "ms": round(sum(ts) / len(ts), 3),
+                             "info_block": st["info_block"], "loo": loo,
+                             "rel_residual": res / xn})), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -407,6 +444,8 @@ def main():
     ap.add_argument("--suite", action="store_true",
                     help="run the per-variant / per-kappa / glued measurement cases "
                          "instead of the headline line (one JSON object per case)")
+    ap.add_argument("--mixed-only", choices=["fp64", "bf16", "fp16", "bf16x3"],
+                    help="run only the mixed form at this first-pass precision (profiling)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="after timing, run one extra plain elementwise copy of X (read 8mn, "
                          "write 8mn bytes) to calibrate HBM byte counters")
@@ -423,6 +462,9 @@ def main():
         return
     if args.batched:
         batched(args)
+        return
+    if args.mixed_only:
+        mixed_only(args)
         return
 
     import torch
@@ -478,19 +520,25 @@ def main():
         acc = {"loo": loo, "rel_residual": res / xn}
         log("LOO %.3e  residual %.3e" % (loo, res / xn))
 
-    # Streaming ceiling of this device: a plain copy X0 -> X (after the
-    # accuracy check: it overwrites the factorization's Q) (8mn bytes read
-    # and 8mn written), best of three, timed with HIP events.
-    copy_ms = []
+    # Streaming ceiling of this device (after the accuracy check: the copy
+    # overwrites the factorization's Q): the library's plain 16-byte copy
+    # X0 -> X (8mn bytes read, 8mn written) and plain read of X0, best of
+    # three each, HIP-event timed; torch's copy beside them.
+    cp_ms = min(B.stream_probe(0, X.data_ptr(), X0.data_ptr(), 8 * m * n) for _ in range(3))
+    rd_ms = min(B.stream_probe(1, 0, X0.data_ptr(), 8 * m * n) for _ in range(3))
+    copy_gbs = 16.0 * m * n / (cp_ms / 1e3) / 1e9
+    read_gbs = 8.0 * m * n / (rd_ms / 1e3) / 1e9
+    tc = []
     for _ in range(3):
         c0, c1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         c0.record(stream)
         X.copy_(X0)
         c1.record(stream)
         torch.cuda.synchronize()
-        copy_ms.append(c0.elapsed_time(c1))
-    copy_gbs = 16.0 * m * n / (min(copy_ms) / 1e3) / 1e9
-    log("copy X0 -> X: %.3f ms, %.1f GB/s" % (min(copy_ms), copy_gbs))
+        tc.append(c0.elapsed_time(c1))
+    torch_copy_gbs = 16.0 * m * n / (min(tc) / 1e3) / 1e9
+    log("streaming ceiling: copy %.1f GB/s, read %.1f GB/s (torch copy %.1f GB/s)" % (
+        copy_gbs, read_gbs, torch_copy_gbs))
 
     # Mixed-precision leg: the same input rounded to FP32 storage; first-pass
     # products in each reduced precision, FP64 reorthogonalization.
@@ -561,7 +609,8 @@ def main():
                  "share_of_step": round(d["ms"] / (ms * args.steps), 3),
                  "alg_gbs": round(ba / per_fact / 1e9, 1),
                  "schedule_tflops": round(sched_tfs, 3), "schedule_gbs": round(sched_gbs, 1),
-                 "copy_gbs": round(copy_gbs, 1)})
+                 "copy_gbs": round(copy_gbs, 1), "read_gbs": round(read_gbs, 1),
+                 "torch_copy_gbs": round(torch_copy_gbs, 1)})
 
     value = b_alg(m, n, s) / (ms / 1e3) / 1e9
     out = {

@@ -183,6 +183,14 @@ int orth_kernel_timing(int enable);
 int orth_kernel_times(int kind, double *total_ms, int *count, double *flops,
                       double *bytes);
 
+/* Streaming ceiling of the device (measurement aid, no counterpart in the
+ * reference): mode 0 times a plain copy of `bytes` from src to dst, mode 1 a
+ * plain read of `bytes` of src, both with 16-byte non-temporal accesses and
+ * eight 256-thread workgroups per CU; *ms_out receives the HIP-event time of
+ * the kernel.  Synchronous (waits for the device first).  Returns 0, or 2
+ * for invalid arguments or a HIP error. */
+int orth_stream_probe(int mode, double *dst, const double *src, int64_t bytes, double *ms_out);
+
 /* Wait for all work the library enqueued on this device. */
 int orth_sync(void);
 

@@ -97,6 +97,8 @@ def lib():
                                                  _i64p, _i64p, _i64p, _vpp, _i64p, _vpp,
                                                  _i64p, _vpp, _i64p, _ip, ctypes.c_int, _ip,
                                                  _vp]
+    L.orth_stream_probe.restype = ctypes.c_int
+    L.orth_stream_probe.argtypes = [ctypes.c_int, _vp, _vp, _i64, _dp]
     if hasattr(L, "orth_bcgs_device_noalloc"):
         L.orth_bcgs_device_noalloc.restype = ctypes.c_int
         L.orth_bcgs_device_noalloc.argtypes = [ctypes.c_int, _i64, _i64, _i64, _vp, _i64, _vp,
@@ -223,6 +225,16 @@ def update_device(m, x_ptr, ldx, col0, w, c, w_ptr, ldw, t_ptr, stream=0):
                                   t_ptr, stream or None)
     if rc != 0:
         raise RuntimeError("orth_update_device returned %d" % rc)
+
+
+def stream_probe(mode, dst_ptr, src_ptr, nbytes):
+    """Milliseconds of one plain streaming copy (mode 0) or read (mode 1) of
+    nbytes on the device."""
+    ms = ctypes.c_double()
+    rc = lib().orth_stream_probe(mode, dst_ptr, src_ptr, nbytes, ctypes.byref(ms))
+    if rc != 0:
+        raise RuntimeError("orth_stream_probe returned %d" % rc)
+    return ms.value
 
 
 def sync():

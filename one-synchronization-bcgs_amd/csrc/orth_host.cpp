@@ -122,7 +122,8 @@ int64_t env_bits()
     static const char *const names[] = {"ORTH_UNFUSED", "ORTH_UPDATE_ROWWISE", "ORTH_PASS256",
                                         "ORTH_GENERIC_PASS", "ORTH_MIXED_PASS32",
                                         "ORTH_PASS_TILE32", "ORTH_PASS_LOAD8",
-                                        "ORTH_SEPARATE_REDUCE"};
+                                        "ORTH_SEPARATE_REDUCE", "ORTH_PASS_REGPF", "ORTH_PASS16D_ONEWG",
+                                        "ORTH_PASS16D_MINCOLS"};
     int64_t b = 0;
     for (int i = 0; i < (int)(sizeof(names) / sizeof(names[0])); i++)
         if (getenv(names[i])) b |= (int64_t)1 << i;
@@ -575,8 +576,10 @@ class Run {
         // 256-thread passes (whose s = 16 form needs the tile padded to
         // ceil(Ma / 4) projection tiles per wave).
         int mt = 0, nbt = 0;
-        const int ts16 =
-            (S_ == 16 && !mx_ && !getenv("ORTH_PASS256")) ? pass16b_plan(pa, &mt, &nbt) : 0;
+        const int td = (S_ == 16 && !mx_ && !getenv("ORTH_PASS256")) ? pass16d_plan(pa, &mt, &nbt) : 0;
+        const int ts16 = td > 0 ? 32
+                         : (S_ == 16 && !mx_ && !getenv("ORTH_PASS256")) ? pass16b_plan(pa, &mt, &nbt)
+                                                                          : 0;
         int64_t unit = kPassRows;
         int tsm = 0;
         if (mx_) {
@@ -607,7 +610,12 @@ class Run {
         // registers); with FP64 products it runs two, like the FP64 pass.
         // The 256-thread passes keep as many workgroups per CU as their
         // registers and LDS allow.
-        const int per_cu = (ts16 > 0 || mx_ == kMxFP64) ? 2 : mx_ ? 1 : pass_wg_per_cu(S_, pa);
+        // The double-buffered pass with 16 waves holds a whole CU (one
+        // workgroup).
+        const int per_cu = td == 16                          ? 1
+                           : (ts16 > 0 || mx_ == kMxFP64) ? 2
+                           : mx_                           ? 1
+                                                           : pass_wg_per_cu(S_, pa);
         int nwg = std::max(1, (int)(std::max(1, c_.ncu) * per_cu * wg_scale_));
         if (na > 0 && sc_.fixed) {
             // Fixed scratch: fewer row chunks, so that the partials fit.
@@ -630,6 +638,7 @@ class Run {
                 if (mx_)
                     return tsm > 0 ? launch_pass16bm(pa, mx_, nwg, st_)
                                    : launch_pass16m(pa, mx_, nwg, st_);
+                if (td > 0) return launch_pass16d(pa, nwg, st_);
                 return ts16 > 0 ? launch_pass16b(pa, nwg, st_) : launch_pass(S_, pa, nwg, st_);
             }))
             return nullptr;
@@ -1842,6 +1851,30 @@ int orth_update_device(int64_t m, double *dX, int64_t ldX, int64_t col0, int64_t
     const bool launched = ok(launch_update(S, dX, ldX, m, &u, 1, st));
     const bool done = ok(hipStreamSynchronize(st));
     return launched && done ? 0 : 2;
+}
+
+int orth_stream_probe(int mode, double *dst, const double *src, int64_t bytes, double *ms_out)
+{
+    if (!src || !ms_out || (mode == 0 && !dst) || mode < 0 || mode > 1 || bytes < 16) return 2;
+    tl_hip_error = 0;
+    const int dev = current_device();
+    DevCtx &c = g_ctx[dev];
+    std::lock_guard<std::mutex> lk(c.mu);
+    if (!ctx_init(c)) return 2;
+    // Eight workgroups per CU (a grid sweep on MI355X showed no gain beyond).
+    const int nwg = 8 * std::max(1, c.ncu);
+    if (mode == 1 && !grow(c.sc, c.sc.part, c.sc.part_n, (size_t)nwg)) return 2;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (!ok(hipEventCreate(&e0)) || !ok(hipEventCreate(&e1))) return 2;
+    bool good = ok(hipDeviceSynchronize()) && ok(hipEventRecord(e0, nullptr)) &&
+                ok(launch_stream_probe(mode, dst, src, bytes, c.sc.part, nwg, nullptr)) &&
+                ok(hipEventRecord(e1, nullptr)) && ok(hipEventSynchronize(e1));
+    float ms = 0.f;
+    good = good && ok(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms_out = ms;
+    return good ? 0 : 2;
 }
 
 int orth_sync(void)

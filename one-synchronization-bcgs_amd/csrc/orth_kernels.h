@@ -175,6 +175,12 @@ int pass16b_plan(const PassArgs &a, int *mt, int *nb);
 size_t pass16b_lds_bytes(const PassArgs &a, int ts);
 hipError_t launch_pass16b(const PassArgs &a, int nwg, hipStream_t st);
 
+// Block width 16, FP64, double-buffered direct-to-LDS loads in 32-row tiles
+// (rows_per must be a multiple of 32): waves per workgroup (8 or 16), or 0
+// when it does not apply.
+int pass16d_plan(const PassArgs &a, int *mt, int *nb);
+hipError_t launch_pass16d(const PassArgs &a, int nwg, hipStream_t st);
+
 // Mixed-precision s = 16 pass (32-row tiles, 512 threads): mx selects the
 // reduced precision of the marked products (1 bf16, 2 fp16, 3 bf16x3 split,
 // 4 none: FP64 arithmetic on FP32-stored input).  pass16m_lds_bytes returns
@@ -187,6 +193,12 @@ hipError_t launch_pass16m(const PassArgs &a, int mx, int nwg, hipStream_t st);
 // per CU): 0 when the shape does not fit it (then launch_pass16m applies).
 int pass16bm_plan(const PassArgs &a);
 hipError_t launch_pass16bm(const PassArgs &a, int mx, int nwg, hipStream_t st);
+
+// Streaming probe: mode 0 copies `bytes` from src to dst, mode 1 reads
+// `bytes` of src (one partial sum per workgroup to part[nwg]); 16-byte
+// non-temporal accesses, nwg workgroups of 256 threads.
+hipError_t launch_stream_probe(int mode, double *dst, const double *src, int64_t bytes,
+                               double *part, int nwg, hipStream_t st);
 
 // Q = X32 widened exactly to FP64 (m x n, column-major).
 hipError_t launch_widen(const float *x, int64_t ldx, double *q, int64_t ldq, int64_t m, int64_t n,

@@ -23,6 +23,7 @@
 
 #include "orth_device.h"
 #include "orth_pass16b.h"
+#include "orth_pass16d.h"
 #include "orth_small.h"
 
 namespace orth {
@@ -1085,6 +1086,89 @@ hipError_t launch_pass16b(const PassArgs &a, int nwg, hipStream_t st)
 #undef ORTH_P16B
 }
 
+// The double-buffered s = 16 pass (orth_pass16d.h) with 32-row tiles: waves
+// per workgroup (8: two workgroups per CU, when both buffers of two
+// workgroups fit a CU's LDS; 16: one workgroup per CU, up to the whole
+// 160 KB), or 0 when it does not apply.  It needs FP64 data with an even
+// row count and leading dimension and a 16-byte aligned slab (16-byte loads
+// of row pairs), the folded (non-sequential) step 1, and more than
+// kPass16dMinCols - 1 columns: measured at m = 2^24, n = 256, it beats
+// k_pass16b on the passes over more than 192 columns (by 0.4-2.3 ms each)
+// and loses on the narrower ones, where k_pass16b runs two workgroups per CU
+// with 64- and 128-row super-tiles (ORTH_PASS16D_MINCOLS overrides the bound).
+constexpr int kPass16dMinCols = 193;
+
+int pass16d_plan(const PassArgs &a, int *mt, int *nb)
+{
+    if (getenv("ORTH_PASS_REGPF")) return 0;
+    if (a.n1 != 32 || a.seq || a.X32 || (a.na > 0 && a.nb > 32)) return 0;
+    // Row pairs are loaded whole: an odd row count would read past the last
+    // row of the last column.
+    if (a.ldX % 2 != 0 || a.m % 2 != 0 || (uintptr_t)a.X % 16 != 0) return 0;
+    int minc = kPass16dMinCols;
+    if (const char *e = getenv("ORTH_PASS16D_MINCOLS")) minc = atoi(e);
+    if (a.ncols < minc || a.ncols > a.wl) return 0;
+    const bool force16 = getenv("ORTH_PASS16D_ONEWG") != nullptr;
+    int nw = 0;
+    if (!force16 && 2 * pass16d_lds(a.wl, 8) <= kDLds) nw = 8;
+    else if (pass16d_lds(a.wl, 16) <= kDLds) nw = 16;
+    if (nw == 0) return 0;
+    const int ks = nw / 4;
+    const int nks = a.k1 > 0 ? ((a.k1 + 3) & ~3) / 4 : 0;
+    if ((nks + ks - 1) / ks > kDKq || (a.k1 > 0 && ((a.k1 + 3) & ~3) > a.wl)) return 0;
+    // Each group's Z columns are one whole Z tile (zoff 0 or 16, distinct):
+    // the wave that sums that tile solves the group.
+    for (int g = 0; g < a.ngrp; g++)
+        if (a.g[g].w > 16 || a.g[g].t0 + a.g[g].w > a.ncols || a.g[g].t0 % 16 != 0 ||
+            a.g[g].t0 + 16 > a.wl || (a.g[g].zoff != 0 && a.g[g].zoff != 16))
+            return 0;
+    if (a.ngrp == 2 && a.g[0].zoff == a.g[1].zoff) return 0;
+    *mt = 1;
+    *nb = 1;
+    if (a.na > 0) {
+        const int Ma = (a.na + 15) / 16, Nb = (a.nb + 15) / 16;
+        if (a.a0 + Ma * 16 > a.wl || a.b0 + Nb * 16 > a.wl) return 0;
+        // The LDS bound (wl <= 128 with 8 waves, <= 256 with 16) keeps Ma
+        // within one M-tile per wave.
+        *mt = (Ma + nw - 1) / nw;
+        *nb = Nb;
+        if (*mt > 1) return 0;
+    }
+    return nw;
+}
+
+template <int NW, int NB>
+hipError_t launch_pass16d_t(const PassArgs &a, int nwg, hipStream_t st)
+{
+    static bool attr_set = false;
+    if (!attr_set) {
+        hipError_t e = hipFuncSetAttribute((const void *)k_pass16d<NW, NB>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)(160 * 1024));
+        if (e != hipSuccess) return e;
+        attr_set = true;
+    }
+    const size_t lds = (size_t)pass16d_lds(a.wl, NW) * sizeof(double);
+    k_pass16d<NW, NB><<<dim3(nwg), dim3(64 * NW), lds, st>>>(a);
+    return hipGetLastError();
+}
+
+template <int NW>
+hipError_t launch_pass16d_nw(const PassArgs &a, int mt, int nb, int nwg, hipStream_t st)
+{
+    if (mt != 1) return hipErrorInvalidValue;
+    return nb == 2 ? launch_pass16d_t<NW, 2>(a, nwg, st) : launch_pass16d_t<NW, 1>(a, nwg, st);
+}
+
+hipError_t launch_pass16d(const PassArgs &a, int nwg, hipStream_t st)
+{
+    int mt = 0, nb = 0;
+    const int nw = pass16d_plan(a, &mt, &nb);
+    if (nw == 0 || a.rows_per % 32 != 0) return hipErrorInvalidValue;
+    return nw == 16 ? launch_pass16d_nw<16>(a, mt, nb, nwg, st)
+                    : launch_pass16d_nw<8>(a, mt, nb, nwg, st);
+}
+
 // Tiles per wave of the s = 16 kernel for a pass, or 0 when it does not apply.
 static int pass16_mt(const PassArgs &a)
 {
@@ -1231,6 +1315,74 @@ hipError_t launch_pass(int S, const PassArgs &a, int nwg, hipStream_t st)
     case 32: return launch_pass_t<32>(a, nwg, st);
     default: return hipErrorInvalidValue;
     }
+}
+
+// ---------------------------------------------------------------------------
+// Streaming probes (measurement of the device's streaming ceiling)
+// ---------------------------------------------------------------------------
+
+typedef double dbl2p __attribute__((ext_vector_type(2)));
+
+// dst = src over n2 16-byte elements, 8 non-temporal loads in flight per
+// thread before the stores.
+__global__ __launch_bounds__(256) void k_probe_copy(const dbl2p *__restrict__ src,
+                                                    dbl2p *__restrict__ dst, int64_t n2)
+{
+    constexpr int U = 8;
+    const int64_t stride = (int64_t)gridDim.x * 256 * U;
+    for (int64_t base = (int64_t)blockIdx.x * 256 * U + threadIdx.x; base < n2; base += stride) {
+        dbl2p v[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int64_t i = base + (int64_t)u * 256;
+            v[u] = i < n2 ? __builtin_nontemporal_load(src + i) : dbl2p{0.0, 0.0};
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int64_t i = base + (int64_t)u * 256;
+            if (i < n2) __builtin_nontemporal_store(v[u], dst + i);
+        }
+    }
+}
+
+// Reads n2 16-byte elements; one per-workgroup sum goes to part[blockIdx.x]
+// (so that the loads are kept).
+__global__ __launch_bounds__(256) void k_probe_read(const dbl2p *__restrict__ src, double *part,
+                                                    int64_t n2)
+{
+    constexpr int U = 8;
+    __shared__ double red[256];
+    const int64_t stride = (int64_t)gridDim.x * 256 * U;
+    double t = 0.0;
+    for (int64_t base = (int64_t)blockIdx.x * 256 * U + threadIdx.x; base < n2; base += stride) {
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int64_t i = base + (int64_t)u * 256;
+            if (i < n2) {
+                const dbl2p v = __builtin_nontemporal_load(src + i);
+                t += v.x + v.y;
+            }
+        }
+    }
+    red[threadIdx.x] = t;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int k = 0; k < 256; k++) s += red[k];
+        part[blockIdx.x] = s;
+    }
+}
+
+hipError_t launch_stream_probe(int mode, double *dst, const double *src, int64_t bytes,
+                               double *part, int nwg, hipStream_t st)
+{
+    const int64_t n2 = bytes / 16;
+    if (n2 < 1 || nwg < 1) return hipErrorInvalidValue;
+    if (mode == 0)
+        k_probe_copy<<<dim3(nwg), dim3(256), 0, st>>>((const dbl2p *)src, (dbl2p *)dst, n2);
+    else
+        k_probe_read<<<dim3(nwg), dim3(256), 0, st>>>((const dbl2p *)src, part, n2);
+    return hipGetLastError();
 }
 
 // Q (m x n, ld ldq) = X32 widened exactly to FP64.

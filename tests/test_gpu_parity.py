@@ -253,6 +253,29 @@ def test_alternate_pass_kernels(variant, env, monkeypatch):
     check_against_oracle(variant, X, 16)
 
 
+@pytest.mark.parametrize("shape", [(70000, 256, 16), (33334, 160, 16), (5000, 96, 16),
+                                   (4096, 250, 16), (778, 48, 16), (40002, 112, 16),
+                                   (40001, 112, 16)])
+@pytest.mark.parametrize("onewg", [False, True])
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_double_buffered_pass(variant, onewg, shape, monkeypatch):
+    """The double-buffered s = 16 pass (direct-to-LDS loads; forced on every
+    pass it applies to, with two 512-thread or one 1024-thread workgroup per
+    CU) against the oracle and against the register-prefetch pass: partial
+    last tiles (m not a multiple of the tile), a ragged last block (n = 250),
+    tiny slabs, and an odd row count (which the double-buffered pass
+    declines)."""
+    m, n, s = shape
+    X = I.svd_matrix(m, n, 1e4, seed=34)
+    monkeypatch.setenv("ORTH_PASS16D_MINCOLS", "16")
+    if onewg:
+        monkeypatch.setenv("ORTH_PASS16D_ONEWG", "1")
+    Qa, Ra, _ = check_against_oracle(variant, X, s)
+    monkeypatch.setenv("ORTH_PASS_REGPF", "1")
+    Qb, Rb, _ = check_against_oracle(variant, X, s)
+    assert np.abs(Ra - Rb).max() <= 1e-12 * np.abs(Ra).max()
+
+
 @pytest.mark.parametrize("shape", [(6000, 80, 16), (5000, 70, 8), (3000, 96, 32), (70000, 256, 16)])
 @pytest.mark.parametrize("variant", VARIANTS)
 def test_separate_reduction_matches_in_kernel(variant, shape, monkeypatch):
