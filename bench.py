@@ -245,7 +245,7 @@ def suite(args):
                       "loo": loo, "rel_residual": res, "info_block": st["info_block"],
                       "passes": st["passes"], "syncs": st["syncs"]})
         X32 = X0.float()
-        for prec in ("fp64", "bf16", "fp16", "bf16x3"):
+        for prec in ("fp64", "bf16", "fp16", "bf16x3", "bf16_proj", "fp16_proj", "bf16x3_proj"):
             ms, st, loo, res = run_mixed(torch, B, prec, X0, X32, m, n, s)
             emit({"config": "mixed", "precision": prec, "m": m, "n": n, "s": s,
                   "kappa": kappa, "ms": round(ms, 3),
@@ -444,7 +444,8 @@ def main():
     ap.add_argument("--suite", action="store_true",
                     help="run the per-variant / per-kappa / glued measurement cases "
                          "instead of the headline line (one JSON object per case)")
-    ap.add_argument("--mixed-only", choices=["fp64", "bf16", "fp16", "bf16x3"],
+    ap.add_argument("--mixed-only", choices=["fp64", "bf16", "fp16", "bf16x3", "bf16_proj",
+                                             "fp16_proj", "bf16x3_proj"],
                     help="run only the mixed form at this first-pass precision (profiling)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="after timing, run one extra plain elementwise copy of X (read 8mn, "
@@ -546,7 +547,7 @@ def main():
     if not args.no_mixed and args.variant == "bcgsi2P1s" and s == 16 and n <= 256:
         X32 = X0.float()
         base_loo = acc.get("loo")
-        for prec in ("fp64", "bf16", "fp16", "bf16x3"):
+        for prec in ("fp64", "bf16", "fp16", "bf16x3", "bf16_proj", "fp16_proj", "bf16x3_proj"):
             ts = []
             for k in range(1 + min(args.steps, 3)):
                 R.zero_()

@@ -34,7 +34,12 @@ enum {
     ORTH_PREC_FP64 = 0,   /* FP64 products; only the input storage is FP32    */
     ORTH_PREC_BF16 = 1,   /* bf16 MFMA operands, FP32 accumulation            */
     ORTH_PREC_FP16 = 2,   /* fp16 MFMA operands, FP32 accumulation            */
-    ORTH_PREC_BF16X3 = 3  /* split bf16 operands (hi*lo + lo*hi + hi*hi)      */
+    ORTH_PREC_BF16X3 = 3, /* split bf16 operands (hi*lo + lo*hi + hi*hi)      */
+    /* The same three for the first-pass projection and Gram only; the
+     * first-pass update X - Q S stays FP64. */
+    ORTH_PREC_BF16_PROJ = 4,
+    ORTH_PREC_FP16_PROJ = 5,
+    ORTH_PREC_BF16X3_PROJ = 6
 };
 
 typedef struct {
@@ -81,7 +86,9 @@ int orth_bcgs_device_noalloc(int variant, int64_t m, int64_t n, int64_t s, doubl
  * (the update X - Q S and the projection Q^T X with the Gram X^T X of a
  * block): ORTH_PREC_BF16 / _FP16 on MFMA with FP32 accumulation per 32-row
  * tile, ORTH_PREC_BF16X3 (FP32 values split into two bf16 terms, three
- * products), or ORTH_PREC_FP64.  The reorthogonalization (second-pass
+ * products), or ORTH_PREC_FP64; the _PROJ precisions reduce only the
+ * projection and Gram and keep the update X - Q S in FP64 (so the residual
+ * stays at the FP32 input's rounding).  The reorthogonalization (second-pass
  * update and projection), the Pythagorean subtraction, the Cholesky factors
  * and the triangular solves are FP64 in every case.  The reduced
  * precisions support variant ORTH_P1S with s = 16 and n <= 256 (the fused

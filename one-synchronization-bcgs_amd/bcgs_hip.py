@@ -22,7 +22,8 @@ VARIANTS = {
     "bcgsi2": 3,
     "bcgsi2P2s": 4,
 }
-PRECISIONS = {"fp64": 0, "bf16": 1, "fp16": 2, "bf16x3": 3}
+PRECISIONS = {"fp64": 0, "bf16": 1, "fp16": 2, "bf16x3": 3,
+              "bf16_proj": 4, "fp16_proj": 5, "bf16x3_proj": 6}
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _fp = ctypes.POINTER(ctypes.c_float)

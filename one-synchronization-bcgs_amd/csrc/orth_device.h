@@ -76,7 +76,13 @@ struct LpFrag {
 };
 
 // Load 8 FP64 values p[0], p[st], ..., p[7 st] from LDS and convert them,
-// four at a time (bounded registers).
+// four at a time (bounded registers).  The values go through FP32 (one
+// v_cvt_f32_f64 each, then a packed FP32 -> bf16 / fp16 conversion): the
+// FP32 value is kept opaque, since a fused FP64 -> bf16 conversion becomes
+// an eight-instruction sequence of FP64 compares and conversions per value,
+// which share the FP64 pipeline with the MFMA.  Rounding twice differs from
+// rounding once only on ties of the reduced format, and not at all for the
+// FP32 input columns.
 template <int MX>
 __device__ __forceinline__ LpFrag<MX> lp_frag(const double *p, int st)
 {
@@ -85,7 +91,10 @@ __device__ __forceinline__ LpFrag<MX> lp_frag(const double *p, int st)
     for (int h = 0; h < 2; h++) {
         float x[4];
 #pragma unroll
-        for (int e = 0; e < 4; e++) x[e] = (float)p[(4 * h + e) * st];
+        for (int e = 0; e < 4; e++) {
+            x[e] = (float)p[(4 * h + e) * st];
+            asm volatile("" : "+v"(x[e]));
+        }
 #pragma unroll
         for (int e = 0; e < 4; e++) {
             if constexpr (MX == kMxFP16) {
@@ -109,7 +118,10 @@ __device__ __forceinline__ LpFrag<MX> lp_frag_g(const double *p, int st, int nva
     for (int h = 0; h < 2; h++) {
         float x[4];
 #pragma unroll
-        for (int e = 0; e < 4; e++) x[e] = 4 * h + e < nvalid ? (float)p[(4 * h + e) * st] : 0.0f;
+        for (int e = 0; e < 4; e++) {
+            x[e] = 4 * h + e < nvalid ? (float)p[(4 * h + e) * st] : 0.0f;
+            asm volatile("" : "+v"(x[e]));
+        }
 #pragma unroll
         for (int e = 0; e < 4; e++) {
             if constexpr (MX == kMxFP16) {

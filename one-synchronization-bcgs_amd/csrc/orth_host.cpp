@@ -518,13 +518,19 @@ class Run {
 
     // Mixed form: the input comes from FP32 storage and the marked
     // first-pass products run in the reduced precision mx (kMx*).
+    // mx may carry kMxProjOnly: then only the first-pass projection and
+    // Gram run in the reduced precision.
     void set_mixed(int mx, const float *x32, int64_t ldx32)
     {
-        mx_ = mx;
+        mx_ = mx & ~kMxProjOnly;
+        proj_only_ = (mx & kMxProjOnly) != 0;
         x32_ = x32;
         ldx32_ = ldx32;
     }
+    // The first-pass projection tiles run in the reduced precision.
     bool lp_first_pass() const { return mx_ != 0 && mx_ != kMxFP64; }
+    // So does the first-pass update.
+    bool lp_update() const { return lp_first_pass() && !proj_only_; }
 
     // Fraction of the device's workgroups one pass uses (the batched form
     // runs several calls side by side).
@@ -575,13 +581,6 @@ class Run {
         // Kernel choice: the 512-thread s = 16 pass when it applies, else the
         // 256-thread passes (whose s = 16 form needs the tile padded to
         // ceil(Ma / 4) projection tiles per wave).
-        int mt = 0, nbt = 0;
-        const int td = (S_ == 16 && !mx_ && !getenv("ORTH_PASS256")) ? pass16d_plan(pa, &mt, &nbt) : 0;
-        const int ts16 = td > 0 ? 32
-                         : (S_ == 16 && !mx_ && !getenv("ORTH_PASS256")) ? pass16b_plan(pa, &mt, &nbt)
-                                                                          : 0;
-        int64_t unit = kPassRows;
-        int tsm = 0;
         if (mx_) {
             // Columns at or past the written frontier still live only in X32.
             pa.X32 = x32_ + cb * ldx32_;
@@ -589,6 +588,19 @@ class Run {
             pa.raw0 = (int)std::max<int64_t>(0, frontier_ - cb);
             if (pa.raw0 < pa.ncols && pa.raw0 % 16 != 0) return nullptr;
             pa.lpb = lp_from_ >= 0 ? (int)(lp_from_ - cb) : (1 << 30);
+        }
+        int mt = 0, nbt = 0;
+        const int td = (S_ == 16 && !getenv("ORTH_PASS256") && !getenv("ORTH_MIXED_PASS32"))
+                           ? pass16d_plan(pa, &mt, &nbt, lp_first_pass())
+                           : 0;
+        const int ts16 = td > 0 ? 32
+                         : (S_ == 16 && !mx_ && !getenv("ORTH_PASS256")) ? pass16b_plan(pa, &mt, &nbt)
+                                                                          : 0;
+        int64_t unit = kPassRows;
+        int tsm = 0;
+        if (td > 0) {
+            unit = 32;
+        } else if (mx_) {
             // The 512-thread pass with super-tiles when the slab allows it,
             // else the 32-row mixed pass.
             tsm = getenv("ORTH_MIXED_PASS32") ? 0 : pass16bm_plan(pa);
@@ -602,7 +614,7 @@ class Run {
             const int64_t mtp = ((na + 15) / 16 + 3) / 4;
             pa.wl = (int)std::max<int64_t>(pa.wl, a0 - cb + 64 * mtp);
         }
-        if (!mx_ && ts16 == 0) unit = pass_rows(S_, pa);
+        if (!mx_ && td == 0 && ts16 == 0) unit = pass_rows(S_, pa);
         // Two 512-thread workgroups per CU when the s = 16 pass applies (its
         // LDS is under half a CU's), so one hides the other's barriers.
         // The mixed pass with reduced-precision products runs one 512-thread
@@ -635,10 +647,11 @@ class Run {
             wcols += pend_.g[g].w;
         }
         if (!timed(c_, ORTH_K_PASS, st_, flops, 8.0 * m_ * (ncols + wcols), [&] {
+                if (td > 0) return mx_ ? launch_pass16d_mixed(pa, mx_, td, nbt, nwg, st_)
+                                       : launch_pass16d(pa, td, nbt, nwg, st_);
                 if (mx_)
                     return tsm > 0 ? launch_pass16bm(pa, mx_, nwg, st_)
                                    : launch_pass16m(pa, mx_, nwg, st_);
-                if (td > 0) return launch_pass16d(pa, nwg, st_);
                 return ts16 > 0 ? launch_pass16b(pa, nwg, st_) : launch_pass(S_, pa, nwg, st_);
             }))
             return nullptr;
@@ -731,7 +744,7 @@ class Run {
         pend_.k1 = (int)(c + wb);
         pend_.seq = seq ? 1 : 0;
         pend_group(c, wb, 0, c, Om_);
-        if (wn > 0) pend_group(c + wb, wn, S_, c + wb, next_T ? Sig_ : nullptr, lp_first_pass());
+        if (wn > 0) pend_group(c + wb, wn, S_, c + wb, next_T ? Sig_ : nullptr, lp_update());
     }
 
     bool p1s_f()
@@ -744,7 +757,7 @@ class Run {
         if (lp_first_pass()) lp_from_ = s_;
         if (!pass_small(0, s_ + w1, s_, w1, sa(OP_OPEN_PYTH, 0, 1, s_, w1, 0, nullptr, s_ + w1))) return false;
         pend_.k1 = (int)s_;
-        pend_group(s_, w1, 0, s_, Sig_, lp_first_pass());
+        pend_group(s_, w1, 0, s_, Sig_, lp_update());
         int64_t c = s_;
         for (int64_t b = 1; b < N_; b++) {
             const int64_t wb = bw(b), wn = (b + 1 < N_) ? bw(b + 1) : 0;
@@ -882,6 +895,7 @@ class Run {
     bool fused_ = false;
     int n1_ = 32;
     int mx_ = 0;                // mixed form: reduced precision (kMx*), 0 = plain FP64
+    bool proj_only_ = false;    // mixed form: the first-pass update stays FP64
     const float *x32_ = nullptr;
     int64_t ldx32_ = 0;
     int64_t frontier_ = 0;      // columns [0, frontier_) have been written to X
@@ -1610,6 +1624,9 @@ int orth_bcgs_mixed_device(int variant, int precision, int64_t m, int64_t n, int
     case ORTH_PREC_BF16: mx = kMxBF16; break;
     case ORTH_PREC_FP16: mx = kMxFP16; break;
     case ORTH_PREC_BF16X3: mx = kMxBF16X3; break;
+    case ORTH_PREC_BF16_PROJ: mx = kMxBF16 | kMxProjOnly; break;
+    case ORTH_PREC_FP16_PROJ: mx = kMxFP16 | kMxProjOnly; break;
+    case ORTH_PREC_BF16X3_PROJ: mx = kMxBF16X3 | kMxProjOnly; break;
     default: return 2;
     }
     tl_hip_error = 0;

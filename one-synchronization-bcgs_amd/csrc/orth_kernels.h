@@ -178,14 +178,24 @@ hipError_t launch_pass16b(const PassArgs &a, int nwg, hipStream_t st);
 // Block width 16, FP64, double-buffered direct-to-LDS loads in 32-row tiles
 // (rows_per must be a multiple of 32): waves per workgroup (8 or 16), or 0
 // when it does not apply.
-int pass16d_plan(const PassArgs &a, int *mt, int *nb);
-hipError_t launch_pass16d(const PassArgs &a, int nwg, hipStream_t st);
+// any_width: also below the column bound (the mixed form with reduced
+// precisions, whose 512-thread pass runs one workgroup per CU).
+int pass16d_plan(const PassArgs &a, int *mt, int *nb, bool any_width = false);
+// Launch with the plan's waves per workgroup (nw) and N-tiles (nb).
+hipError_t launch_pass16d(const PassArgs &a, int nw, int nb, int nwg, hipStream_t st);
+// Its mixed form (a.X32 set): mx = kMxFP64 (FP32 storage, FP64 products) or
+// a reduced precision for the projection tiles from lpb on.
+hipError_t launch_pass16d_mixed(const PassArgs &a, int mx, int nw, int nb, int nwg,
+                                hipStream_t st);
 
 // Mixed-precision s = 16 pass (32-row tiles, 512 threads): mx selects the
 // reduced precision of the marked products (1 bf16, 2 fp16, 3 bf16x3 split,
 // 4 none: FP64 arithmetic on FP32-stored input).  pass16m_lds_bytes returns
 // 0 when the shape does not fit it.
 enum { kMxBF16 = 1, kMxFP16 = 2, kMxBF16X3 = 3, kMxFP64 = 4 };
+// Host flag beside a reduced precision: only the first-pass projection and
+// Gram are reduced, the first-pass update stays FP64.
+constexpr int kMxProjOnly = 8;
 size_t pass16m_lds_bytes(const PassArgs &a);
 hipError_t launch_pass16m(const PassArgs &a, int mx, int nwg, hipStream_t st);
 

@@ -1091,21 +1091,29 @@ hipError_t launch_pass16b(const PassArgs &a, int nwg, hipStream_t st)
 // workgroups fit a CU's LDS; 16: one workgroup per CU, up to the whole
 // 160 KB), or 0 when it does not apply.  It needs FP64 data with an even
 // row count and leading dimension and a 16-byte aligned slab (16-byte loads
-// of row pairs), the folded (non-sequential) step 1, and more than
-// kPass16dMinCols - 1 columns: measured at m = 2^24, n = 256, it beats
-// k_pass16b on the passes over more than 192 columns (by 0.4-2.3 ms each)
-// and loses on the narrower ones, where k_pass16b runs two workgroups per CU
-// with 64- and 128-row super-tiles (ORTH_PASS16D_MINCOLS overrides the bound).
-constexpr int kPass16dMinCols = 193;
+// of row pairs), the folded (non-sequential) step 1, and at least
+// kPass16dMinCols columns: measured at m = 2^24, n = 256, it beats k_pass16b
+// on the passes over 128 columns and more (by up to 0.8 ms each) and loses
+// on the narrower ones, where k_pass16b runs two workgroups per CU with 64-
+// to 256-row super-tiles (ORTH_PASS16D_MINCOLS overrides the bound).
+constexpr int kPass16dMinCols = 128;
 
-int pass16d_plan(const PassArgs &a, int *mt, int *nb)
+int pass16d_plan(const PassArgs &a, int *mt, int *nb, bool any_width)
 {
     if (getenv("ORTH_PASS_REGPF")) return 0;
-    if (a.n1 != 32 || a.seq || a.X32 || (a.na > 0 && a.nb > 32)) return 0;
+    if (a.n1 != 32 || a.seq || (a.na > 0 && a.nb > 32)) return 0;
+    if (a.X32) {
+        // Mixed form: at most 32 raw (FP32) columns, none of them a step-1
+        // operand, a 16-column aligned frontier, no reduced-precision update.
+        const int ndma = std::min(a.ncols, a.raw0);
+        if (a.raw0 % 16 != 0 || a.ncols - ndma > 32 || ((a.k1 + 3) & ~3) > ndma) return 0;
+        for (int g = 0; g < a.ngrp; g++)
+            if (a.g[g].lp) return 0;
+    }
     // Row pairs are loaded whole: an odd row count would read past the last
     // row of the last column.
     if (a.ldX % 2 != 0 || a.m % 2 != 0 || (uintptr_t)a.X % 16 != 0) return 0;
-    int minc = kPass16dMinCols;
+    int minc = any_width ? 1 : kPass16dMinCols;
     if (const char *e = getenv("ORTH_PASS16D_MINCOLS")) minc = atoi(e);
     if (a.ncols < minc || a.ncols > a.wl) return 0;
     const bool force16 = getenv("ORTH_PASS16D_ONEWG") != nullptr;
@@ -1142,14 +1150,14 @@ hipError_t launch_pass16d_t(const PassArgs &a, int nwg, hipStream_t st)
 {
     static bool attr_set = false;
     if (!attr_set) {
-        hipError_t e = hipFuncSetAttribute((const void *)k_pass16d<NW, NB>,
+        hipError_t e = hipFuncSetAttribute((const void *)k_pass16d<NW, NB, 0>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize,
                                            (int)(160 * 1024));
         if (e != hipSuccess) return e;
         attr_set = true;
     }
     const size_t lds = (size_t)pass16d_lds(a.wl, NW) * sizeof(double);
-    k_pass16d<NW, NB><<<dim3(nwg), dim3(64 * NW), lds, st>>>(a);
+    k_pass16d<NW, NB, 0><<<dim3(nwg), dim3(64 * NW), lds, st>>>(a);
     return hipGetLastError();
 }
 
@@ -1160,13 +1168,10 @@ hipError_t launch_pass16d_nw(const PassArgs &a, int mt, int nb, int nwg, hipStre
     return nb == 2 ? launch_pass16d_t<NW, 2>(a, nwg, st) : launch_pass16d_t<NW, 1>(a, nwg, st);
 }
 
-hipError_t launch_pass16d(const PassArgs &a, int nwg, hipStream_t st)
+hipError_t launch_pass16d(const PassArgs &a, int nw, int nb, int nwg, hipStream_t st)
 {
-    int mt = 0, nb = 0;
-    const int nw = pass16d_plan(a, &mt, &nb);
-    if (nw == 0 || a.rows_per % 32 != 0) return hipErrorInvalidValue;
-    return nw == 16 ? launch_pass16d_nw<16>(a, mt, nb, nwg, st)
-                    : launch_pass16d_nw<8>(a, mt, nb, nwg, st);
+    if ((nw != 8 && nw != 16) || a.rows_per % 32 != 0) return hipErrorInvalidValue;
+    return nw == 16 ? launch_pass16d_nw<16>(a, 1, nb, nwg, st) : launch_pass16d_nw<8>(a, 1, nb, nwg, st);
 }
 
 // Tiles per wave of the s = 16 kernel for a pass, or 0 when it does not apply.

@@ -21,6 +21,7 @@
 
 #include "orth_device.h"
 #include "orth_pass16b.h"
+#include "orth_pass16d.h"
 
 namespace orth {
 
@@ -368,6 +369,44 @@ hipError_t launch_pass16bm(const PassArgs &a, int mx, int nwg, hipStream_t st)
     case kMxFP16: return launch_bm_mx<kMxFP16>(a, ts, mt, nb, lds, nwg, st);
     case kMxBF16X3: return launch_bm_mx<kMxBF16X3>(a, ts, mt, nb, lds, nwg, st);
     case kMxFP64: return launch_bm_mx<kMxFP64>(a, ts, mt, nb, lds, nwg, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <int NW, int NB, int MX>
+hipError_t launch_pass16d_mixed_t(const PassArgs &a, int nwg, hipStream_t st)
+{
+    static bool attr_set = false;
+    if (!attr_set) {
+        hipError_t e = hipFuncSetAttribute((const void *)k_pass16d<NW, NB, MX>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)(160 * 1024));
+        if (e != hipSuccess) return e;
+        attr_set = true;
+    }
+    const size_t lds = (size_t)pass16d_lds(a.wl, NW) * sizeof(double);
+    k_pass16d<NW, NB, MX><<<dim3(nwg), dim3(64 * NW), lds, st>>>(a);
+    return hipGetLastError();
+}
+
+template <int MX>
+hipError_t launch_pass16d_mx(const PassArgs &a, int nw, int nb, int nwg, hipStream_t st)
+{
+    if (nw == 16)
+        return nb == 2 ? launch_pass16d_mixed_t<16, 2, MX>(a, nwg, st)
+                       : launch_pass16d_mixed_t<16, 1, MX>(a, nwg, st);
+    return nb == 2 ? launch_pass16d_mixed_t<8, 2, MX>(a, nwg, st)
+                   : launch_pass16d_mixed_t<8, 1, MX>(a, nwg, st);
+}
+
+hipError_t launch_pass16d_mixed(const PassArgs &a, int mx, int nw, int nb, int nwg, hipStream_t st)
+{
+    if ((nw != 8 && nw != 16) || a.rows_per % 32 != 0 || !a.X32) return hipErrorInvalidValue;
+    switch (mx) {
+    case kMxFP64: return launch_pass16d_mx<kMxFP64>(a, nw, nb, nwg, st);
+    case kMxBF16: return launch_pass16d_mx<kMxBF16>(a, nw, nb, nwg, st);
+    case kMxFP16: return launch_pass16d_mx<kMxFP16>(a, nw, nb, nwg, st);
+    case kMxBF16X3: return launch_pass16d_mx<kMxBF16X3>(a, nw, nb, nwg, st);
     default: return hipErrorInvalidValue;
     }
 }

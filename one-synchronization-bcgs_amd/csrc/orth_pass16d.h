@@ -82,11 +82,21 @@ __device__ __forceinline__ void lds_barrier()
     asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
-template <int NW, int NB>
+// MX = 0: the FP64 form.  MX != 0 (mixed form): columns from raw0 on (at
+// most 32, never step-1 operands) have not been written yet and come from the
+// FP32 input X32; they are loaded into registers together with the next
+// tile's DMA and written to LDS, widened, when that tile opens (a barrier
+// separates that write from their first reader).  With MX a reduced precision
+// (kMxBF16, kMxFP16, kMxBF16X3), the projection products whose B columns
+// start at or past lpb run as one 16 x 16 x 32 MFMA per 32-row tile with FP32
+// accumulation, summed across tiles in FP64; every other product is FP64.
+template <int NW, int NB, int MX>
 __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_pass16d(
     PassArgs a)
 {
     constexpr int TS = 32, N1 = 32, kD = 64 * NW, LDc = kDLd;
+    constexpr bool MIX = MX != 0, LPP = MX != 0 && MX != kMxFP64;
+    constexpr int MXL = LPP ? MX : kMxBF16;  // operand format of the reduced-precision products
     // Step-1 work items: 2 x 2 Z tiles of 16 x 16, the inner dimension
     // split KS ways so that the items fill the NW waves.
     constexpr int ZT = 4, KS = NW / ZT;
@@ -132,6 +142,13 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4, 4)))
     double bw[4];
 #pragma unroll
     for (int b = 0; b < 4; b++) bw[b] = Tq ? Tq[kSlotW + 128 * b + lk * kLdT + li] : 0.0;
+    // Make the compiler wait for these loads here, once: a use it could not
+    // prove complete inside the row loop would get a wait there, on every
+    // tile, which would also wait for the next tile's loads.
+#pragma unroll
+    for (int j = 0; j < kDKq; j++) asm volatile("" ::"v"(cf[j]));
+#pragma unroll
+    for (int b = 0; b < 4; b++) asm volatile("" ::"v"(bw[b]));
 
     // Projection: wave w owns M-tile w (the host keeps Ma <= NW; waves past
     // Ma have none) and all NB N-tiles.  The operands of k-step ks (row
@@ -141,6 +158,11 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4, 4)))
     // 4..7.
     const int Ma = (a.na + 15) >> 4;
     const bool proj = a.na > 0 && wave < Ma;
+    // Reduced-precision N-tiles; their accumulators hold rows lk 4 + v.
+    bool lowt[NB];
+#pragma unroll
+    for (int t = 0; t < NB; t++) lowt[t] = LPP && a.b0 + 16 * t >= a.lpb;
+    const bool anylow = LPP && lowt[NB - 1];
     dbl4 acc[NB];
 #pragma unroll
     for (int t = 0; t < NB; t++) acc[t] = dbl4{0.0, 0.0, 0.0, 0.0};
@@ -152,15 +174,18 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4, 4)))
     const int64_t re = min(a.m, rb + a.rows_per);
     // Loads: slot span i (64 slots) goes to wave i % NW.  Slot q of column c
     // holds row pair q ^ 8 (c & 1) (the half swap); the pad slot repeats
-    // slot 15's row pair.
-    const int nspan = (a.ncols * kDSl + 63) / 64;
+    // slot 15's row pair.  Mixed form: the DMA covers the columns below raw0.
+    const int ndma = MIX ? min(a.ncols, a.raw0) : a.ncols;
+    const int nspan = (ndma * kDSl + 63) / 64;
     const int64_t rlast = (a.m - 1) & ~(int64_t)1;  // last row-pair start
-    auto issue = [&](int64_t r0, double *buf) {
+    // Spans of this wave: i = wave + NW j; those with j in [j0, j1) are
+    // issued by one call.
+    auto issue = [&](int64_t r0, double *buf, int j0, int j1) {
         const int ln = opaque(lane);
-        for (int i = wave; i < nspan; i += NW) {
+        for (int i = wave + NW * j0; i < nspan && i < wave + NW * j1; i += NW) {
             const int s = i * 64 + ln, c = s / kDSl;
             const int q = min(s - c * kDSl, TS / 2 - 1) ^ ((c & 1) << 3);
-            if (c < a.ncols) {
+            if (c < ndma) {
                 const int64_t row = min(r0 + 2 * q, rlast);
                 const double *g = a.X + row + (int64_t)c * a.ldX;
                 __builtin_amdgcn_global_load_lds((glb_void_t *)g, (lds_void_t *)(buf + i * 128),
@@ -168,7 +193,30 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4, 4)))
             }
         }
     };
-    if (rb < re) issue(rb, sm);
+    // Raw FP32 columns [ndma, ncols): value k of a thread is row e % 32 of
+    // column ndma + e / 32, e = tid + k kD; rows past the matrix are loaded
+    // from the last row and never used.
+    constexpr int EPR = MIX ? (32 * TS + kD - 1) / kD : 1;
+    const int nraw = a.ncols - ndma;
+    float rv[EPR];
+#pragma unroll
+    for (int k = 0; k < EPR; k++) rv[k] = 0.f;
+    auto raw_issue = [&](int64_t r0) {
+        if constexpr (MIX) {
+#pragma unroll
+            for (int k = 0; k < EPR; k++) {
+                const int e = opaque(tid) + k * kD, c = e >> 5;
+                if (c < nraw) {
+                    const int64_t row = min(r0 + (e & 31), a.m - 1);
+                    rv[k] = __builtin_nontemporal_load(a.X32 + row + (int64_t)(ndma + c) * a.ldX32);
+                }
+            }
+        }
+    };
+    if (rb < re) {
+        issue(rb, sm, 0, 1 << 20);
+        raw_issue(rb);
+    }
 
     int it = 0;
     for (int64_t r0 = rb; r0 < re; r0 += TS, it++) {
@@ -178,7 +226,35 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4, 4)))
         // makes all of them visible), and every wave is done with the other
         // buffer: stream tile i+1 into it.
         asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)\n\ts_barrier" ::: "memory");
-        if (r0 + TS < re) issue(r0 + TS, nxt);
+        if constexpr (MIX) {
+            // This tile's raw columns, widened (rows past the slab: zero),
+            // before any load of the next tile is issued (a wait for the
+            // registers then waits for nothing in flight).
+#pragma unroll
+            for (int k = 0; k < EPR; k++) {
+                const int e = opaque(tid) + k * kD, c = e >> 5, r = e & 31;
+                if (c < nraw) cur[dpos(ndma + c, r)] = r0 + r < re ? (double)rv[k] : 0.0;
+            }
+            // With pending groups their solve, after two barriers, is the
+            // first reader of these columns; without, the projection reads
+            // them next.
+            if (nraw > 0 && a.ngrp == 0) lds_barrier();
+        }
+        // The next tile's loads: by the waves that solve a group chunk in
+        // this tile, and by all when no group is pending or no projection
+        // follows the solve, at once; the other waves issue theirs while the
+        // groups are solved, when they have nothing else to do (issuing
+        // blocks a wave while the address path is busy: measured at the
+        // widest benchmark pass, 11.2 ms against 12.3 ms with every wave
+        // issuing at once, 12.0 ms with the non-solving waves issuing all
+        // spans, 11.8 ms with the solving waves issuing theirs after the
+        // solve; without a projection the late loads had too little time).
+        const bool more = r0 + TS < re;
+        const int jl = a.ngrp > 0 && a.na > 0 && !solver ? 0 : 1 << 20;
+        if (more) {
+            issue(r0 + TS, nxt, 0, jl);
+            raw_issue(r0 + TS);
+        }
         const int tq = opaque(tid), lq = opaque(lane);
         const int liq = lq & 15, lkq = lq >> 4;
         if (r0 + TS > re) {
@@ -210,6 +286,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4, 4)))
                 for (int v = 0; v < 4; v++) zs[((ksp - 1) * ZT + item) * 256 + v * 64 + lq] = z[v];
             }
             lds_barrier();
+            if (more) issue(r0 + TS, nxt, jl, 1 << 20);
             if (solver) {
                 // Chunk zm of the group in MFMA accumulator layout (lane:
                 // column t0 + liq, rows 16 zm + lkq + 4 v), minus the
@@ -270,21 +347,37 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4, 4)))
 
         // Step 2: projection partial over this tile's rows (the NB products
         // of a k-step share one A operand read).
-        if (proj) {
+        if (proj && !(anylow && lowt[0])) {
             const double *pa0 = cur + opaque(pab + sa), *pa1 = cur + opaque(pab - sa);
             const double *pb0 = cur + opaque(pbb + sb), *pb1 = cur + opaque(pbb - sb);
 #pragma unroll
             for (int ks = 0; ks < TS / 4; ks++) {
                 const double *pa = ks < 4 ? pa0 : pa1;
                 const double *pb = ks < 4 ? pb0 : pb1;
-                double bv[NB];
-#pragma unroll
-                for (int t = 0; t < NB; t++) bv[t] = pb[4 * ks + 16 * LDc * t];
                 const double av = pa[4 * ks];
 #pragma unroll
                 for (int t = 0; t < NB; t++)
-                    acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv[t], acc[t], 0, 0, 0);
+                    if (!lowt[t])
+                        acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, pb[4 * ks + 16 * LDc * t],
+                                                                      acc[t], 0, 0, 0);
                 __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+        if constexpr (LPP) {
+            if (proj && anylow) {
+                // Operands: rows 8 lk .. 8 lk + 7 of column a0 + 16 w + li
+                // (A) and b0 + 16 t + li (B), eight consecutive doubles.
+                const LpFrag<MXL> fa =
+                    lp_frag<MXL>(cur + opaque(pab - lk + 8 * lk + (lk < 2 ? sa : -sa)), 1);
+#pragma unroll
+                for (int t = 0; t < NB; t++)
+                    if (lowt[t]) {
+                        const LpFrag<MXL> fb = lp_frag<MXL>(
+                            cur + opaque(pbb - lk + 8 * lk + (lk < 2 ? sb : -sb)) + 16 * LDc * t, 1);
+                        const f32x4 tt = lp_mma<MXL>(fa, fb, f32x4{0.f, 0.f, 0.f, 0.f});
+#pragma unroll
+                        for (int v = 0; v < 4; v++) acc[t][v] += (double)tt[v];
+                    }
             }
         }
     }
@@ -296,7 +389,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4, 4)))
             const int col = t * 16 + li;
 #pragma unroll
             for (int v = 0; v < 4; v++) {
-                const int row = wave * 16 + lk + 4 * v;
+                const int row = wave * 16 + (lowt[t] ? 4 * lk + v : lk + 4 * v);
                 if (row < a.na && col < a.nb) out[row + (int64_t)col * a.na] = acc[t][v];
             }
         }

@@ -3,6 +3,9 @@
 * FP64 products on FP32-stored input: same schedule and arithmetic as the
   FP64 form, so Q and R match the CPU oracle run on the FP32-rounded input to
   the FP64 parity tolerance (1e-11 relative in R).
+* Reduced precision for the first-pass projection and Gram only (the _proj
+  precisions): the residual stays at FP64 level and the loss of
+  orthogonality within 10x of the oracle's.
 * bf16 / fp16 / bf16x3 first-pass products: the FP64 second pass restores
   orthogonality, so at kappa = 1e2 the loss of orthogonality stays within 10x
   of the oracle's FP64 value; the first-pass update error stays in the
@@ -51,14 +54,20 @@ def mixed(X, s, precision, variant="bcgsi2P1s", ld=None, pads=None):
     return Q, R, st, X32.astype(np.float64)
 
 
-KERNELS = ["pass512", "pass32"]
+KERNELS = ["pass512", "pass32", "dbuf"]
 
 
 def pick(kernel, monkeypatch):
-    """pass512: the 512-thread pass with super-tiles (default); pass32: the
-    32-row mixed pass (also the fallback for odd leading dimensions)."""
+    """pass512: the 512-thread pass with super-tiles (default below 193
+    columns); pass32: the 32-row mixed pass (also the fallback for odd leading
+    dimensions); dbuf: the double-buffered pass forced on every slab it
+    applies to (FP64 products and reduced-precision projections; passes with
+    a reduced-precision update, odd row counts and odd leading dimensions
+    fall back to the 512-thread pass)."""
     if kernel == "pass32":
         monkeypatch.setenv("ORTH_MIXED_PASS32", "1")
+    if kernel == "dbuf":
+        monkeypatch.setenv("ORTH_PASS16D_MINCOLS", "16")
 
 
 @pytest.mark.parametrize("kernel", KERNELS)
@@ -112,6 +121,57 @@ def test_reduced_precision_first_pass(precision, kernel, monkeypatch):
     assert 0.1 * lvl <= res <= 10 * lvl, (res, lvl)
     # R's upper triangle stays close to the FP64 factorization at that level.
     assert np.abs(R - Ro).max() <= 100 * lvl * np.abs(Ro).max()
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+@pytest.mark.parametrize("precision", ["bf16_proj", "fp16_proj", "bf16x3_proj"])
+@pytest.mark.parametrize("kappa", [1e2, 1e6])
+def test_reduced_precision_projection_only(kappa, precision, kernel, monkeypatch):
+    """Reduced precision for the first-pass projection and Gram only: the
+    first-pass update stays FP64, so X = QR holds to FP64 rounding of the
+    FP32-rounded input (R carries the first pass's projection as computed
+    and the reorthogonalization's correction), and the FP64 second pass
+    keeps the loss of orthogonality within 10x of the oracle's FP64 value.
+    The Pythagorean first-pass factor chol(X^T X - S^T S) cancels to
+    sigma_min^2 / sigma_max^2 of the block, which a reduced-precision S
+    (bf16, fp16, even bf16x3) cannot resolve at kappa = 1e6: there a
+    breakdown may be reported (and Q is not trusted)."""
+    pick(kernel, monkeypatch)
+    m, n = 65536, 128
+    X = I.svd_matrix(m, n, kappa, seed=45)
+    say("case mixed projection-only", precision, kernel, kappa)
+    Q, R, st, Xr = mixed(X, 16, precision)
+    assert st["hip_error"] == 0
+    if kappa > 1e4 and st["info_block"] != 0:
+        return
+    assert st["info_block"] == 0
+    Qo, Ro, _ = O.bcgs("bcgsi2P1s", Xr, 16)
+    loo, loo_o = I.loo(Q), I.loo(Qo)
+    assert loo <= 10 * max(loo_o, 1e-14), (loo, loo_o)
+    res = I.rel_residual(Xr, Q, R)
+    assert res <= 1e-14, res
+    assert np.all(np.tril(R, -1) == 0)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "bf16_proj"])
+@pytest.mark.parametrize("shape", [(70000, 256), (33334, 208), (8192, 96)])
+def test_double_buffered_mixed_matches_512_thread_pass(shape, precision, monkeypatch):
+    """The double-buffered mixed pass (forced on every slab, including the
+    intra-block QR's Gram of raw columns with no pending group) against the
+    512-thread mixed pass on the same input: the reduced-precision tile
+    products are the same 32-row MFMAs in both, so R agrees to the FP64
+    rounding of their different summation orders."""
+    m, n = shape
+    X = I.svd_matrix(m, n, 1e4 if precision == "fp64" else 1e2, seed=46)
+    say("case mixed dbuf vs 512", shape, precision)
+    monkeypatch.setenv("ORTH_PASS16D_MINCOLS", "16")
+    Qa, Ra, sta, Xr = mixed(X, 16, precision)
+    monkeypatch.setenv("ORTH_PASS_REGPF", "1")
+    Qb, Rb, stb, _ = mixed(X, 16, precision)
+    assert sta["hip_error"] == 0 and stb["hip_error"] == 0
+    assert sta["info_block"] == 0 and stb["info_block"] == 0
+    assert np.abs(Ra - Rb).max() <= 1e-11 * np.abs(Rb).max()
+    assert np.abs(Qa - Qb).max() <= 1e-10
 
 
 @pytest.mark.parametrize("precision", ["fp64", "bf16"])
