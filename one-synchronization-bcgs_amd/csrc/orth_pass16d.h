@@ -248,7 +248,8 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4, 4)))
         // widest benchmark pass, 11.2 ms against 12.3 ms with every wave
         // issuing at once, 12.0 ms with the non-solving waves issuing all
         // spans, 11.8 ms with the solving waves issuing theirs after the
-        // solve; without a projection the late loads had too little time).
+        // solve, 12.5 ms with one span issued after every third step-1
+        // MFMA; without a projection the late loads had too little time).
         const bool more = r0 + TS < re;
         const int jl = a.ngrp > 0 && a.na > 0 && !solver ? 0 : 1 << 20;
         if (more) {
