@@ -201,7 +201,9 @@ int orth_stream_probe(int mode, double *dst, const double *src, int64_t bytes, d
 /* Wait for all work the library enqueued on this device. */
 int orth_sync(void);
 
-/* Release the library's device scratch on this device. */
+/* Release the library's device scratch on this device, including the
+ * batched form's captured graphs and per-stream scratch (the next call
+ * allocates and captures again). */
 int orth_release(void);
 
 #ifdef __cplusplus

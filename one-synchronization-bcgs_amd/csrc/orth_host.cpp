@@ -1687,6 +1687,9 @@ int orth_bcgs_batched_mixed_device(int variant, int precision, int count, const 
     case ORTH_PREC_BF16: a.mx = kMxBF16; break;
     case ORTH_PREC_FP16: a.mx = kMxFP16; break;
     case ORTH_PREC_BF16X3: a.mx = kMxBF16X3; break;
+    case ORTH_PREC_BF16_PROJ: a.mx = kMxBF16 | kMxProjOnly; break;
+    case ORTH_PREC_FP16_PROJ: a.mx = kMxFP16 | kMxProjOnly; break;
+    case ORTH_PREC_BF16X3_PROJ: a.mx = kMxBF16X3 | kMxProjOnly; break;
     default: return 2;
     }
     return batched_entry(a, info, stream, tune, config);
@@ -1909,13 +1912,31 @@ int orth_release(void)
     DevCtx &c = g_ctx[dev];
     std::lock_guard<std::mutex> lk(c.mu);
     if (c.done_valid) (void)hipEventSynchronize(c.done);
-    double **bufs[] = {&c.sc.part, &c.sc.P[0], &c.sc.P[1], &c.sc.Rd, &c.sc.Tbig, &c.sc.Cbuf,
-                       &c.Xd};
-    for (double **b : bufs) {
-        if (*b) (void)hipFree(*b);
-        *b = nullptr;
+    auto free_scratch = [](DevCtx &x) {
+        double **bufs[] = {&x.sc.part, &x.sc.P[0], &x.sc.P[1], &x.sc.Rd, &x.sc.Tbig, &x.sc.Cbuf,
+                           &x.Xd};
+        for (double **b : bufs) {
+            if (*b) (void)hipFree(*b);
+            *b = nullptr;
+        }
+        x.sc.part_n = x.sc.P_n = x.sc.Rd_n = x.sc.Tbig_n = x.sc.Cbuf_n = x.Xd_n = 0;
+    };
+    free_scratch(c);
+    // The batched form's captured graphs and per-stream scratch (the batch
+    // calls hold c.mu as well).
+    BatchCtx &b = g_batch[dev];
+    if (b.init) {
+        for (int k = 0; k < kMaxSlots; k++)
+            if (b.st[k]) (void)hipStreamSynchronize(b.st[k]);
+        if (b.main) (void)hipStreamSynchronize(b.main);
+        drop_graphs(b);
+        for (int k = 0; k < kMaxSlots; k++) free_scratch(b.slot[k]);
+        if (b.info_d) (void)hipFree(b.info_d);
+        if (b.info_h) (void)hipHostFree(b.info_h);
+        b.info_d = nullptr;
+        b.info_h = nullptr;
+        b.info_n = 0;
     }
-    c.sc.part_n = c.sc.P_n = c.sc.Rd_n = c.sc.Tbig_n = c.sc.Cbuf_n = c.Xd_n = 0;
     return 0;
 }
 
