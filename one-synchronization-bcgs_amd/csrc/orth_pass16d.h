@@ -193,22 +193,28 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4, 4)))
             }
         }
     };
-    // Raw FP32 columns [ndma, ncols): value k of a thread is row e % 32 of
-    // column ndma + e / 32, e = tid + k kD; rows past the matrix are loaded
-    // from the last row and never used.
-    constexpr int EPR = MIX ? (32 * TS + kD - 1) / kD : 1;
+    // Raw FP32 columns [ndma, ncols): thread e < 8 nraw moves rows
+    // 4 (e % 8) .. 4 (e % 8) + 3 of column ndma + e / 8, with one 16-byte
+    // load when the FP32 slab is 16-byte aligned with a leading dimension
+    // that is a multiple of 4 (a quarter of the load instructions of 4-byte
+    // loads: the address path is what limits the pass), else with four
+    // 4-byte loads; rows past the matrix are loaded from the last row and
+    // never used.
+    typedef float flt4 __attribute__((ext_vector_type(4)));
     const int nraw = a.ncols - ndma;
-    float rv[EPR];
-#pragma unroll
-    for (int k = 0; k < EPR; k++) rv[k] = 0.f;
+    const bool rv4 = MIX && a.ldX32 % 4 == 0 && (uintptr_t)a.X32 % 16 == 0;
+    flt4 rv = flt4{0.f, 0.f, 0.f, 0.f};
     auto raw_issue = [&](int64_t r0) {
         if constexpr (MIX) {
+            const int e = opaque(tid), c = e >> 3;
+            if (c < nraw) {
+                const int64_t row = r0 + 4 * (e & 7);
+                const float *p = a.X32 + (int64_t)(ndma + c) * a.ldX32;
+                if (rv4 && row + 3 < a.m) {
+                    rv = __builtin_nontemporal_load((const flt4 *)(p + row));
+                } else {
 #pragma unroll
-            for (int k = 0; k < EPR; k++) {
-                const int e = opaque(tid) + k * kD, c = e >> 5;
-                if (c < nraw) {
-                    const int64_t row = min(r0 + (e & 31), a.m - 1);
-                    rv[k] = __builtin_nontemporal_load(a.X32 + row + (int64_t)(ndma + c) * a.ldX32);
+                    for (int q = 0; q < 4; q++) rv[q] = __builtin_nontemporal_load(p + min(row + q, a.m - 1));
                 }
             }
         }
@@ -230,10 +236,13 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4, 4)))
             // This tile's raw columns, widened (rows past the slab: zero),
             // before any load of the next tile is issued (a wait for the
             // registers then waits for nothing in flight).
+            {
+                const int e = opaque(tid), c = e >> 3, r4 = 4 * (e & 7);
+                if (c < nraw) {
+                    double *d = cur + dpos(ndma + c, r4);
 #pragma unroll
-            for (int k = 0; k < EPR; k++) {
-                const int e = opaque(tid) + k * kD, c = e >> 5, r = e & 31;
-                if (c < nraw) cur[dpos(ndma + c, r)] = r0 + r < re ? (double)rv[k] : 0.0;
+                    for (int q = 0; q < 4; q++) d[q] = r0 + r4 + q < re ? (double)rv[q] : 0.0;
+                }
             }
             // With pending groups their solve, after two barriers, is the
             // first reader of these columns; without, the projection reads
