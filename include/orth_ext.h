@@ -138,6 +138,8 @@ int orth_bcgs_batched_mixed_device(int variant, int precision, int count,
 
 /* Accuracy harness of the reference (SRC/util.c:36-75) on device data:
  * out[0] = ||Q^T Q - I||_F, out[1] = ||Q R - X||_F, out[2] = ||X||_F.
+ * Q^T Q is accumulated with compensated (double-double) arithmetic, so its
+ * own rounding stays far below the loss of orthogonality it measures.
  * Synchronous; out is host memory. */
 int orth_accuracy_device(int64_t m, int64_t n, const double *dX, int64_t ldX,
                          const double *dQ, int64_t ldQ, const double *dR,

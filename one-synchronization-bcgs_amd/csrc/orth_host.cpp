@@ -1734,36 +1734,51 @@ int orth_accuracy_device(int64_t m, int64_t n, const double *dX, int64_t ldX, co
 {
     if (!out || m < 1 || n < 1) return 2;
     hipStream_t st = (hipStream_t)stream;
-    double *G = nullptr;
-    if (!ok(hipMalloc(&G, (size_t)n * n * sizeof(double)))) return 2;
-    int rc = orth_gram_device(m, dQ, ldQ, 0, n, 0, n, G, n, stream);
-    if (rc != 0) {
-        (void)hipFree(G);
-        return rc;
-    }
-    std::vector<double> Gh((size_t)n * n);
+    // Q^T Q with compensated accumulation (per row chunk and 64 x 64 block
+    // of the upper block triangle), summed over the chunks in double-double
+    // on the host in a fixed order.
+    const int nch = gram_dd_chunks(m);
+    const int64_t nb = (n + 63) / 64, nblk = nb * (nb + 1) / 2;
+    const size_t npart = (size_t)nch * nblk * 256 * 32;
+    double *part = nullptr;
+    if (!ok(hipMalloc(&part, npart * sizeof(double)))) return 2;
     const int nwg = resid_wg_count(m);
     double *part2 = nullptr;
     if (!ok(hipMalloc(&part2, (size_t)2 * nwg * sizeof(double)))) {
-        (void)hipFree(G);
+        (void)hipFree(part);
         return 2;
     }
-    std::vector<double> ph((size_t)2 * nwg);
-    bool good = ok(launch_resid(m, n, dX, ldX, dQ, ldQ, dR, ldR, part2, st)) &&
-                ok(hipMemcpyAsync(Gh.data(), G, Gh.size() * sizeof(double),
+    std::vector<double> gp(npart), ph((size_t)2 * nwg);
+    bool good = ok(launch_gram_dd(dQ, ldQ, m, n, part, st)) &&
+                ok(launch_resid(m, n, dX, ldX, dQ, ldQ, dR, ldR, part2, st)) &&
+                ok(hipMemcpyAsync(gp.data(), part, npart * sizeof(double),
                                   hipMemcpyDeviceToHost, st)) &&
                 ok(hipMemcpyAsync(ph.data(), part2, ph.size() * sizeof(double),
                                   hipMemcpyDeviceToHost, st)) &&
                 ok(hipStreamSynchronize(st));
-    (void)hipFree(G);
+    (void)hipFree(part);
     (void)hipFree(part2);
     if (!good) return 2;
+    // ||Q^T Q - I||_F^2 over the pairs i <= j (off-diagonal ones twice).
     double loo = 0.0;
-    for (int64_t j = 0; j < n; j++)
-        for (int64_t i = 0; i < n; i++) {
-            const double v = Gh[i + j * n] - (i == j ? 1.0 : 0.0);
-            loo += v * v;
-        }
+    int64_t blk = 0;
+    for (int64_t I = 0; I < nb; I++)
+        for (int64_t J = I; J < nb; J++, blk++)
+            for (int t = 0; t < 256; t++)
+                for (int k = 0; k < 16; k++) {
+                    const int64_t i = 64 * I + 4 * (t % 16) + k / 4;
+                    const int64_t j = 64 * J + 4 * (t / 16) + k % 4;
+                    if (i >= n || j >= n || i > j) continue;
+                    double H = 0.0, L = 0.0;
+                    for (int c = 0; c < nch; c++) {
+                        const double *g = &gp[(((size_t)c * nblk + blk) * 256 + t) * 32 + 2 * k];
+                        const double s = H + g[0], bp = s - H;
+                        L += (H - (s - bp)) + (g[0] - bp) + g[1];
+                        H = s;
+                    }
+                    const double v = (i == j ? H - 1.0 : H) + L;
+                    loo += (i == j ? 1.0 : 2.0) * v * v;
+                }
     double sr = 0.0, sx = 0.0;
     for (int k = 0; k < nwg; k++) {
         sr += ph[2 * k];

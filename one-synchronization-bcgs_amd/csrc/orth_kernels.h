@@ -219,6 +219,15 @@ hipError_t launch_widen(const float *x, int64_t ldx, double *q, int64_t ldq, int
 hipError_t launch_fill(double *p, size_t n, hipStream_t st);
 hipError_t launch_fill_int(int *p, size_t n, hipStream_t st);
 
+// Accuracy harness: Q^T Q with compensated (double-double) accumulation.
+// part receives, per row chunk (gram_dd_chunks(m) of them) and 64 x 64
+// block of the upper block triangle (nb (nb + 1) / 2, nb = ceil(n / 64)), 256
+// threads x 16 pairs x (hi, lo): pair k = 4 di + dj of thread t is
+// (64 I + 4 (t % 16) + di, 64 J + 4 (t / 16) + dj).
+int gram_dd_chunks(int64_t m);
+hipError_t launch_gram_dd(const double *Q, int64_t ldQ, int64_t m, int64_t n, double *part,
+                          hipStream_t st);
+
 // Residual harness: per-workgroup partial sums of ||Q R - X||^2 and ||X||^2
 // into part2[2*x], part2[2*x+1]; returns the number of workgroups used.
 int resid_wg_count(int64_t m);

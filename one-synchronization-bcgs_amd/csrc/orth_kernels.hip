@@ -1504,4 +1504,89 @@ hipError_t launch_resid(int64_t m, int64_t n, const double *X, int64_t ldX, cons
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Accuracy harness: the Gram Q^T Q with compensated accumulation (Dot2 of
+// Ogita, Rump and Oishi, "Accurate sum and dot product", SIAM J. Sci. Comput.
+// 26 (2005): TwoProduct by FMA, TwoSum, the error terms summed in FP64), so
+// that the Gram's own rounding stays far below the loss of orthogonality it
+// measures.  Workgroup (x, y): row chunk x, 64 x 64 block y of the upper
+// block triangle; thread t owns the 16 pairs (I0 + 4 (t % 16) + di,
+// J0 + 4 (t / 16) + dj).  part: (hi, lo) of each pair, per chunk and block.
+// ---------------------------------------------------------------------------
+constexpr int kGramRows = 32;
+
+__global__ __launch_bounds__(256) void k_gram_dd(const double *__restrict__ Q, int64_t ldQ,
+                                                 int64_t m, int n, int64_t rows_per,
+                                                 double *__restrict__ part)
+{
+#pragma clang fp contract(off)
+    __shared__ double sa[kGramRows][65], sb[kGramRows][65];
+    const int t = threadIdx.x;
+    const int nb = (n + 63) / 64;
+    // Block y -> (I, J), I <= J, row-major over the upper block triangle.
+    int y = blockIdx.y, I = 0;
+    while (y >= nb - I) {
+        y -= nb - I;
+        I++;
+    }
+    const int J = I + y, I0 = 64 * I, J0 = 64 * J;
+    const int ti = 4 * (t % 16), tj = 4 * (t / 16);
+    double hi[16], lo[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) hi[k] = lo[k] = 0.0;
+    const int64_t rb = (int64_t)blockIdx.x * rows_per, re = min(m, rb + rows_per);
+    for (int64_t r0 = rb; r0 < re; r0 += kGramRows) {
+        for (int e = t; e < kGramRows * 64; e += 256) {
+            const int c = e / kGramRows, r = e % kGramRows;
+            const bool in = r0 + r < re;
+            sa[r][c] = in && I0 + c < n ? Q[r0 + r + (int64_t)(I0 + c) * ldQ] : 0.0;
+            sb[r][c] = in && J0 + c < n ? Q[r0 + r + (int64_t)(J0 + c) * ldQ] : 0.0;
+        }
+        __syncthreads();
+        for (int r = 0; r < kGramRows; r++) {
+            double a[4], b[4];
+#pragma unroll
+            for (int d = 0; d < 4; d++) {
+                a[d] = sa[r][ti + d];
+                b[d] = sb[r][tj + d];
+            }
+#pragma unroll
+            for (int di = 0; di < 4; di++)
+#pragma unroll
+                for (int dj = 0; dj < 4; dj++) {
+                    const int k = 4 * di + dj;
+                    const double p = a[di] * b[dj];
+                    const double pe = __builtin_fma(a[di], b[dj], -p);
+                    const double sum = hi[k] + p;
+                    const double bp = sum - hi[k];
+                    const double se = (hi[k] - (sum - bp)) + (p - bp);
+                    hi[k] = sum;
+                    lo[k] += se + pe;
+                }
+        }
+        __syncthreads();
+    }
+    double *o = part + ((int64_t)(blockIdx.x * gridDim.y + blockIdx.y) * 256 + t) * 32;
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        o[2 * k] = hi[k];
+        o[2 * k + 1] = lo[k];
+    }
+}
+
+int gram_dd_chunks(int64_t m)
+{
+    return (int)std::max<int64_t>(1, std::min<int64_t>(64, (m + 4095) / 4096));
+}
+
+hipError_t launch_gram_dd(const double *Q, int64_t ldQ, int64_t m, int64_t n, double *part,
+                          hipStream_t st)
+{
+    const int nch = gram_dd_chunks(m), nb = (int)((n + 63) / 64);
+    int64_t rows = (m + nch - 1) / nch;
+    rows = (rows + kGramRows - 1) / kGramRows * kGramRows;
+    k_gram_dd<<<dim3(nch, nb * (nb + 1) / 2), dim3(256), 0, st>>>(Q, ldQ, m, (int)n, rows, part);
+    return hipGetLastError();
+}
+
 }  // namespace orth

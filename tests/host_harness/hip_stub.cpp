@@ -464,6 +464,14 @@ int resid_wg_count(int64_t)
 {
     return 64;
 }
+int gram_dd_chunks(int64_t m)
+{
+    return m >= 4096 ? 64 : 1;
+}
+hipError_t launch_gram_dd(const double *, int64_t, int64_t, int64_t, double *, hipStream_t st)
+{
+    return stub_launch(st);
+}
 hipError_t launch_resid(int64_t, int64_t, const double *, int64_t, const double *, int64_t,
                         const double *, int64_t, double *part2, hipStream_t st)
 {

@@ -9,6 +9,8 @@ Cholesky breakdown is reported for the same block as the oracle.
 import os
 import sys
 
+import math
+
 import numpy as np
 import pytest
 
@@ -174,6 +176,41 @@ def test_gram_and_accuracy_device():
     ref = O.accuracy(X, Qn, R)
     for g_, r_ in zip(got, ref):
         assert abs(g_ - r_) <= 1e-8 * r_
+
+
+def test_accuracy_gram_is_compensated():
+    """The harness's Q^T Q is accumulated in double-double: on columns whose
+    exact Gram is known in closed form and whose ||Q^T Q - I|| (~1e-14) is
+    far below the rounding of an FP64 accumulation over m = 2^20 rows
+    (~sqrt(m) u ~ 1e-13 per entry), the harness returns the exact value.
+    Column 0: 2^-10 (1 + s_r 2^-30), column 1: 2^-10 e_r (1 + t_r 2^-30),
+    e_r = +-1, with sum(s) = sum(t) = sum(e) = sum(e (s + t)) = 0, so that
+    G00 - 1 = 2^-80 sum(s^2), G11 - 1 = 2^-80 sum(t^2) and
+    G01 = 2^-80 sum(e s t) exactly (every entry is exact in FP64)."""
+    m = 1 << 20
+    h = m // 2
+    rng = np.random.default_rng(29)
+    s1 = rng.integers(-64, 65, h, dtype=np.int64)
+    t1 = rng.integers(-64, 65, h, dtype=np.int64)
+    e1 = rng.permutation(np.repeat(np.array([1, -1], dtype=np.int64), h // 2))
+    # Second half: (s, t) of the first half negated with the same signs e
+    # (balanced), so that every first-order sum vanishes while the products
+    # e s t add up.
+    s = np.concatenate([s1, -s1])
+    t = np.concatenate([t1, -t1])
+    e = np.concatenate([e1, e1])
+    assert s.sum() == 0 and t.sum() == 0 and e.sum() == 0 and (e * (s + t)).sum() == 0
+    q0 = 2.0 ** -10 * (1.0 + s.astype(np.float64) * 2.0 ** -30)
+    q1 = 2.0 ** -10 * e.astype(np.float64) * (1.0 + t.astype(np.float64) * 2.0 ** -30)
+    Q = np.stack([q0, q1], axis=1)
+    ss, tt, est = int((s * s).sum()), int((t * t).sum()), int((e * s * t).sum())
+    exact = math.sqrt(float(ss * ss + 2 * est * est + tt * tt)) * 2.0 ** -80
+    say("case compensated gram", exact)
+    Qd = torch.from_numpy(np.ascontiguousarray(Q.T)).cuda()
+    Rd = torch.eye(2, dtype=torch.float64, device="cuda")
+    loo, res, xn = B.accuracy_device(m, 2, Qd.data_ptr(), m, Qd.data_ptr(), m, Rd.data_ptr(), 2)
+    assert abs(loo - exact) <= 1e-12 * exact, (loo, exact)
+    assert res == 0.0
 
 
 @pytest.mark.skipif(torch is None, reason="torch missing")
