@@ -577,7 +577,6 @@ class Run {
         pa.na = (int)na;
         pa.b0 = (int)(b0 - cb);
         pa.nb = (int)nb;
-        pa.part = sc_.part;
         // Kernel choice: the 512-thread s = 16 pass when it applies, else the
         // 256-thread passes (whose s = 16 form needs the tile padded to
         // ceil(Ma / 4) projection tiles per wave).
@@ -629,9 +628,18 @@ class Run {
                            : mx_                           ? 1
                                                            : pass_wg_per_cu(S_, pa);
         int nwg = std::max(1, (int)(std::max(1, c_.ncu) * per_cu * wg_scale_));
+        // The pass synchronizes itself (reduction and small step in its last
+        // workgroups) when the scratch has the ticket counters; its group
+        // sums follow the partials in `part`.
+        bool tail = na > 0 && sc_.tickets && !getenv("ORTH_SEPARATE_REDUCE") &&
+                    !getenv("ORTH_REDUCE_LAUNCH");
         if (na > 0 && sc_.fixed) {
             // Fixed scratch: fewer row chunks, so that the partials fit.
-            const int64_t cap = (int64_t)(sc_.part_n / ((size_t)na * nb));
+            int64_t cap = (int64_t)(sc_.part_n / ((size_t)na * nb));
+            if (tail && cap > 2 * kTailMaxGroups)
+                cap -= kTailMaxGroups;
+            else
+                tail = false;
             if (cap < 1) return nullptr;
             nwg = (int)std::min<int64_t>(nwg, cap);
         }
@@ -639,7 +647,25 @@ class Run {
         rows = (rows + unit - 1) / unit * unit;
         pa.rows_per = rows;
         nwg = (int)((m_ + rows - 1) / rows);
-        if (na > 0 && !grow(sc_, sc_.part, sc_.part_n, (size_t)nwg * na * nb)) return nullptr;
+        const int ngrp = tail ? (nwg + tail_group(nwg) - 1) / tail_group(nwg) : 0;
+        if (na > 0 && !grow(sc_, sc_.part, sc_.part_n, (size_t)(nwg + ngrp) * na * nb))
+            return nullptr;
+        pa.part = sc_.part;
+        double *Pn = na > 0 ? sc_.P[pcur_ ^ 1] : nullptr;
+        if (tail) {
+            pa.tail.on = 1;
+            pa.tail.nwg = nwg;
+            pa.tail.grp = tail_group(nwg);
+            pa.tail.tickets = sc_.tickets;
+            pa.tail.lvl = sc_.part + (size_t)nwg * na * nb;
+            pa.tail.out = Pn;
+            if (after) {
+                pa.tail.small = 1;
+                pa.tail.sm = *after;
+                pa.tail.sm.P = Pn;
+                finish_small(pa.tail.sm);
+            }
+        }
         double flops = 2.0 * m_ * na * nb, wcols = 0.0;
         for (int g = 0; g < pend_.ngrp; g++) {
             const double kk = pend_.seq ? pend_.g[g].k : pend_.k1;
@@ -662,9 +688,12 @@ class Run {
         passes_++;
         if (na == 0) return sc_.P[pcur_];
         pcur_ ^= 1;
-        double *P = sc_.P[pcur_];
         syncs_++;
-        return sync(nwg, na, nb, P, after) ? P : nullptr;
+        if (tail) {
+            if (debug_ && after) dump(pa.tail.sm);
+            return Pn;
+        }
+        return sync(nwg, na, nb, Pn, after) ? Pn : nullptr;
     }
 
     // The synchronization after a projection: P <- the fixed-order sum of the

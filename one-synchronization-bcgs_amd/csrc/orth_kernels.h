@@ -84,6 +84,38 @@ struct SmallArgs {
 hipError_t launch_update(int S, double *X, int64_t ldX, int64_t m,
                          const Upd *u, int nupd, hipStream_t st);
 
+// Fixed summation order of the projection partials, shared by every
+// reduction (launch_reduce, launch_reduce_small and the in-pass tail): the
+// nwg partials form groups of tail_group(nwg) consecutive workgroups; each
+// group is summed in workgroup order starting from its first partial, and
+// the group sums are added in group order starting from the first.  At most
+// kTailMaxGroups groups.
+constexpr int kTailMaxGroups = 32;
+inline int tail_group(int nwg)
+{
+    int g = 1;
+    while (g * g < nwg) g++;
+    return g * kTailMaxGroups < nwg ? (nwg + kTailMaxGroups - 1) / kTailMaxGroups : g;
+}
+
+// The synchronization at the end of a fused pass (the reference's
+// MPI_Allreduce and the small step after it, SRC/orth.c:114, :133, :188,
+// :197), run inside the pass launch by tickets, without waiting: the last
+// workgroup of each group to finish sums that group's partials into lvl, the
+// last of those sums the group sums into out and runs the small step.  The
+// counters are set back to zero by the workgroups that read them last.
+struct TailArgs {
+    int on;        // 1: the pass reduces its own partials
+    int small;     // 1: then runs `sm` (sm.P = out) in its last workgroup
+    int nwg;       // workgroups of the pass
+    int grp;       // tail_group(nwg)
+    int *tickets;  // [0, kTailMaxGroups) group counters, [kTailTop] the top counter
+    double *lvl;   // group sums: ceil(nwg / grp) x (na nb) doubles
+    double *out;   // the sum, na x nb, ld na
+    SmallArgs sm;
+};
+constexpr int kTailTop = kTailMaxGroups;
+
 // Projection partials: part[x][i + j*na] = sum over the rows of workgroup x
 // of A(:, i) * B(:, j), A = columns [a0, a0+na), B = columns [b0, b0+nb).
 struct ProjPlan {
@@ -155,7 +187,13 @@ struct PassArgs {
     int64_t ldX32;
     int raw0;
     int lpb;
+    TailArgs tail;
 };
+
+// LDS bytes a pass needs at least when its tail runs a small step (the
+// small step's region and the ticket flag); the launchers raise their
+// dynamic LDS to it.
+size_t pass_tail_lds(const PassArgs &a, size_t lds);
 
 constexpr int kPassRows = 32;
 constexpr int kPassMaxCols = 288;

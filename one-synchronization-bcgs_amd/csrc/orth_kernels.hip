@@ -445,32 +445,35 @@ hipError_t launch_proj(const ProjPlan &pl, const double *X, int64_t ldX, int64_t
     return hipGetLastError();
 }
 
-__global__ __launch_bounds__(256) void k_reduce(const double *__restrict__ part, int nwg,
+// Sums of the partials of 32 elements in the fixed order of the in-pass tail
+// (tail_group): thread (el, q) sums groups q, q + 8, ... of element
+// blockIdx.x * 32 + el into red, then q == 0 adds the group sums in order.
+static __device__ double reduce_fixed_order(const double *__restrict__ part, int nwg, int grp,
+                                            int64_t E, double (*red)[32])
+{
+    const int el = threadIdx.x & 31, q = threadIdx.x >> 5;
+    const int64_t e = (int64_t)blockIdx.x * 32 + el;
+    const int ngrp = (nwg + grp - 1) / grp;
+    if (e < E)
+        for (int g = q; g < ngrp; g += 8)
+            red[g][el] = part_range_sum(part, E, e, g * grp, min(nwg, g * grp + grp));
+    __syncthreads();
+    double t = 0.0;
+    if (q == 0 && e < E) {
+        t = red[0][el];
+        for (int g = 1; g < ngrp; g++) t += red[g][el];
+    }
+    return t;
+}
+
+__global__ __launch_bounds__(256) void k_reduce(const double *__restrict__ part, int nwg, int grp,
                                                 int64_t na, int64_t E,
                                                 double *__restrict__ out, int64_t ldo)
 {
-    __shared__ double red[8][32];
-    const int el = threadIdx.x & 31, g = threadIdx.x >> 5;
-    const int64_t e = (int64_t)blockIdx.x * 32 + el;
-    // Four independent partial sums (four loads in flight per thread),
-    // combined in a fixed order: the result is the same on every run.
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-    if (e < E) {
-        int w = g;
-        for (; w + 24 < nwg; w += 32) {
-            s0 += part[(int64_t)w * E + e];
-            s1 += part[(int64_t)(w + 8) * E + e];
-            s2 += part[(int64_t)(w + 16) * E + e];
-            s3 += part[(int64_t)(w + 24) * E + e];
-        }
-        for (; w < nwg; w += 8) s0 += part[(int64_t)w * E + e];
-    }
-    red[g][el] = (s0 + s1) + (s2 + s3);
-    __syncthreads();
-    if (g == 0 && e < E) {
-        double t = 0.0;
-#pragma unroll
-        for (int k = 0; k < 8; k++) t += red[k][el];
+    __shared__ double red[kTailMaxGroups][32];
+    const double t = reduce_fixed_order(part, nwg, grp, E, red);
+    const int64_t e = (int64_t)blockIdx.x * 32 + (threadIdx.x & 31);
+    if ((threadIdx.x >> 5) == 0 && e < E) {
         const int64_t i = e % na, j = e / na;
         out[i + j * ldo] = t;
     }
@@ -481,7 +484,8 @@ hipError_t launch_reduce(const double *part, int nwg, int64_t na, int64_t nb, do
 {
     const int64_t E = na * nb;
     if (E <= 0) return hipSuccess;
-    k_reduce<<<dim3((unsigned)((E + 31) / 32)), dim3(256), 0, st>>>(part, nwg, na, E, out, ldo);
+    k_reduce<<<dim3((unsigned)((E + 31) / 32)), dim3(256), 0, st>>>(part, nwg, tail_group(nwg), na,
+                                                                   E, out, ldo);
     return hipGetLastError();
 }
 
@@ -489,33 +493,15 @@ hipError_t launch_reduce(const double *part, int nwg, int64_t na, int64_t nb, do
 // k_reduce's fixed-order sum, then the small step in the last workgroup to
 // finish (ticket counter; release by every workgroup, acquire by the last).
 __global__ __launch_bounds__(256) void k_reduce_small(const double *__restrict__ part, int nwg,
-                                                      int64_t na, int64_t E, double *out,
+                                                      int grp, int64_t E, double *out,
                                                       int *ticket, SmallArgs sm)
 {
-    __shared__ double red[8][32];
+    __shared__ double red[kTailMaxGroups][32];
     __shared__ double lds[kSmallLds];
     __shared__ int last;
-    const int el = threadIdx.x & 31, g = threadIdx.x >> 5;
-    const int64_t e = (int64_t)blockIdx.x * 32 + el;
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-    if (e < E) {
-        int w = g;
-        for (; w + 24 < nwg; w += 32) {
-            s0 += part[(int64_t)w * E + e];
-            s1 += part[(int64_t)(w + 8) * E + e];
-            s2 += part[(int64_t)(w + 16) * E + e];
-            s3 += part[(int64_t)(w + 24) * E + e];
-        }
-        for (; w < nwg; w += 8) s0 += part[(int64_t)w * E + e];
-    }
-    red[g][el] = (s0 + s1) + (s2 + s3);
-    __syncthreads();
-    if (g == 0 && e < E) {
-        double t = 0.0;
-#pragma unroll
-        for (int k = 0; k < 8; k++) t += red[k][el];
-        out[e] = t;
-    }
+    const double t = reduce_fixed_order(part, nwg, grp, E, red);
+    const int64_t e = (int64_t)blockIdx.x * 32 + (threadIdx.x & 31);
+    if ((threadIdx.x >> 5) == 0 && e < E) out[e] = t;
     // Release this workgroup's outputs, take a ticket; the last workgroup
     // acquires everyone's and runs the small step.
     __threadfence();
@@ -525,7 +511,7 @@ __global__ __launch_bounds__(256) void k_reduce_small(const double *__restrict__
     if (!last) return;
     __threadfence();
     if (threadIdx.x == 0) *ticket = 0;
-    small_step(sm, lds);
+    small_step((int)blockDim.x, sm, lds);
 }
 
 hipError_t launch_reduce_small(const double *part, int nwg, int64_t na, int64_t nb, double *out,
@@ -533,8 +519,8 @@ hipError_t launch_reduce_small(const double *part, int nwg, int64_t na, int64_t 
 {
     const int64_t E = na * nb;
     if (E <= 0) return hipErrorInvalidValue;
-    k_reduce_small<<<dim3((unsigned)((E + 31) / 32)), dim3(256), 0, st>>>(part, nwg, na, E, out,
-                                                                         ticket, sm);
+    k_reduce_small<<<dim3((unsigned)((E + 31) / 32)), dim3(256), 0, st>>>(
+        part, nwg, tail_group(nwg), E, out, ticket, sm);
     return hipGetLastError();
 }
 
@@ -545,7 +531,7 @@ hipError_t launch_reduce_small(const double *part, int nwg, int64_t na, int64_t 
 __global__ __launch_bounds__(256) void k_small(SmallArgs a)
 {
     __shared__ double lds[kSmallLds];
-    small_step(a, lds);
+    small_step((int)blockDim.x, a, lds);
 }
 
 hipError_t launch_small(const SmallArgs &a, hipStream_t st)
@@ -596,6 +582,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
 {
     constexpr int TS = kPT * F, PLD = TS + 1;
     extern __shared__ double sm[];
+    const uint64_t kst = kernarg_stash();
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int li = lane & 15, lk = lane >> 4;
     const int k1p = (a.k1 + 3) & ~3;
@@ -797,6 +784,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(
             }
         }
     }
+    pass_tail(a, sm, kst, 256);
 }
 
 // Block width 16, the benchmark shape: compile-time tile counts, 32-bit
@@ -808,6 +796,7 @@ __global__ __launch_bounds__(256) void k_pass16(PassArgs a)
 {
     constexpr int S = 16, N1 = 32, LD = kPLd;
     extern __shared__ double sm[];
+    const uint64_t kst = kernarg_stash();
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int li = lane & 15, lk = lane >> 4;
     const int k1p = (a.k1 + 3) & ~3;
@@ -984,6 +973,7 @@ __global__ __launch_bounds__(256) void k_pass16(PassArgs a)
             }
         }
     }
+    pass_tail(a, sm, kst, 256);
 }
 
 int pass16b_rows(const PassArgs &a)
@@ -1157,7 +1147,7 @@ hipError_t launch_pass16d_t(const PassArgs &a, int nwg, hipStream_t st)
         attr_set = true;
     }
     const size_t lds = (size_t)pass16d_lds(a.wl, NW) * sizeof(double);
-    k_pass16d<NW, NB, 0><<<dim3(nwg), dim3(64 * NW), lds, st>>>(a);
+    k_pass16d<NW, NB, 0><<<dim3(nwg), dim3(64 * NW), pass_tail_lds(a, lds), st>>>(a);
     return hipGetLastError();
 }
 
@@ -1197,7 +1187,7 @@ hipError_t launch_pass16_t(const PassArgs &a, size_t lds, int nwg, hipStream_t s
         if (e != hipSuccess) return e;
         attr_set = true;
     }
-    k_pass16<MT, NB><<<dim3(nwg), dim3(256), lds, st>>>(a);
+    k_pass16<MT, NB><<<dim3(nwg), dim3(256), pass_tail_lds(a, lds), st>>>(a);
     return hipGetLastError();
 }
 
@@ -1242,7 +1232,7 @@ hipError_t launch_pass_v(const PassArgs &a, size_t lds, int nwg, hipStream_t st)
         if (e != hipSuccess) return e;
         attr_set = true;
     }
-    k_pass<S, MT, PF, V2, F><<<dim3(nwg), dim3(256), lds, st>>>(a);
+    k_pass<S, MT, PF, V2, F><<<dim3(nwg), dim3(256), pass_tail_lds(a, lds), st>>>(a);
     return hipGetLastError();
 }
 
@@ -1302,8 +1292,14 @@ int pass_wg_per_cu(int S, const PassArgs &a)
     // workgroup per CU).
     const bool p16 = S == 16 && pass16_mt(a) > 0 && !getenv("ORTH_GENERIC_PASS");
     const int by_regs = p16 ? 1 : pass_narrow(a) ? pass_waves(S, 4) : pass_waves(S, kPassMaxTiles);
-    const int by_lds = (int)std::max<size_t>(1, (160 * 1024) / lds);
+    // The in-pass tail raises small LDS footprints to its own.
+    const int by_lds = (int)std::max<size_t>(1, (160 * 1024) / std::max(lds, kTailLdsBytes));
     return std::max(1, std::min(by_regs, by_lds));
+}
+
+size_t pass_tail_lds(const PassArgs &a, size_t lds)
+{
+    return a.tail.on ? std::max(lds, kTailLdsBytes) : lds;
 }
 
 hipError_t launch_pass(int S, const PassArgs &a, int nwg, hipStream_t st)

@@ -33,6 +33,7 @@ __global__ __launch_bounds__(kM) __attribute__((amdgpu_waves_per_eu(MX == kMxFP6
 {
     constexpr int N1 = 32, TS = 32, LD = TS + 1, PSTEP = kM / TS;
     extern __shared__ double sm[];
+    const uint64_t kst = kernarg_stash();
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int li = lane & 15, lk = lane >> 4;
     const int k1p = (a.k1 + 3) & ~3;
@@ -244,6 +245,7 @@ __global__ __launch_bounds__(kM) __attribute__((amdgpu_waves_per_eu(MX == kMxFP6
             }
         }
     }
+    pass_tail(a, sm, kst, kM);
 }
 
 size_t pass16m_lds_bytes(const PassArgs &a)
@@ -268,7 +270,7 @@ hipError_t launch_pass16m_t(const PassArgs &a, size_t lds, int nwg, hipStream_t 
         if (e != hipSuccess) return e;
         attr_set = true;
     }
-    k_pass16m<NB, MX><<<dim3(nwg), dim3(kM), lds, st>>>(a);
+    k_pass16m<NB, MX><<<dim3(nwg), dim3(kM), pass_tail_lds(a, lds), st>>>(a);
     return hipGetLastError();
 }
 
@@ -385,7 +387,7 @@ hipError_t launch_pass16d_mixed_t(const PassArgs &a, int nwg, hipStream_t st)
         attr_set = true;
     }
     const size_t lds = (size_t)pass16d_lds(a.wl, NW) * sizeof(double);
-    k_pass16d<NW, NB, MX><<<dim3(nwg), dim3(64 * NW), lds, st>>>(a);
+    k_pass16d<NW, NB, MX><<<dim3(nwg), dim3(64 * NW), pass_tail_lds(a, lds), st>>>(a);
     return hipGetLastError();
 }
 

@@ -5,6 +5,7 @@
 #include <type_traits>
 
 #include "orth_device.h"
+#include "orth_tail.h"
 
 namespace orth {
 
@@ -48,6 +49,7 @@ __global__ __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(
     constexpr int N1 = 32, TS = 32 * F, LD = TS + 1;
     constexpr int LTS = F == 1 ? 5 : F == 2 ? 6 : F == 4 ? 7 : 8;
     extern __shared__ double sm[];
+    const uint64_t kst = kernarg_stash();
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int li = lane & 15, lk = lane >> 4;
     const int k1p = (a.k1 + 3) & ~3;
@@ -476,6 +478,7 @@ __global__ __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(
             }
         }
     }
+    pass_tail(a, sm, kst, kB);
 }
 
 template <int MT, int NB, int F, bool EARLY, bool V2, int MX = 0>
@@ -489,7 +492,7 @@ hipError_t launch_pass16b_v(const PassArgs &a, size_t lds, int nwg, hipStream_t 
         if (e != hipSuccess) return e;
         attr_set = true;
     }
-    k_pass16b<MT, NB, F, EARLY, V2, MX><<<dim3(nwg), dim3(kB), lds, st>>>(a);
+    k_pass16b<MT, NB, F, EARLY, V2, MX><<<dim3(nwg), dim3(kB), pass_tail_lds(a, lds), st>>>(a);
     return hipGetLastError();
 }
 

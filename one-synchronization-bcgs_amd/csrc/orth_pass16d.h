@@ -33,6 +33,7 @@
 #pragma once
 
 #include "orth_device.h"
+#include "orth_tail.h"
 
 namespace orth {
 
@@ -101,6 +102,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4, 4)))
     // split KS ways so that the items fill the NW waves.
     constexpr int ZT = 4, KS = NW / ZT;
     extern __shared__ double sm[];
+    const uint64_t kst = kernarg_stash();
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int li = lane & 15, lk = lane >> 4;
@@ -404,6 +406,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4, 4)))
             }
         }
     }
+    pass_tail(a, sm, kst, 64 * NW);
 }
 
 }  // namespace orth

@@ -316,15 +316,43 @@ def test_double_buffered_pass(variant, onewg, shape, monkeypatch):
 @pytest.mark.parametrize("shape", [(6000, 80, 16), (5000, 70, 8), (3000, 96, 32), (70000, 256, 16)])
 @pytest.mark.parametrize("variant", VARIANTS)
 def test_separate_reduction_matches_in_kernel(variant, shape, monkeypatch):
-    """The passes' in-kernel reduction tree (with the small step run by its
-    last workgroup) against separate reduction and small-step launches:
-    both agree with the oracle, and with each other to rounding."""
+    """The synchronization inside the pass launch (ticket tree over the
+    workgroups' partials, small step in the last workgroup) against the same
+    sum and small step in one separate launch (ORTH_REDUCE_LAUNCH) and in two
+    (ORTH_SEPARATE_REDUCE): all agree with the oracle, and bit for bit with
+    each other (one summation order; each small-step entry is computed by one
+    thread in a fixed order whatever the workgroup size)."""
     m, n, s = shape
     X = I.svd_matrix(m, n, 1e4, seed=33)
     Qa, Ra, _ = check_against_oracle(variant, X, s)
-    monkeypatch.setenv("ORTH_SEPARATE_REDUCE", "1")
+    monkeypatch.setenv("ORTH_REDUCE_LAUNCH", "1")
     Qb, Rb, _ = check_against_oracle(variant, X, s)
-    assert np.abs(Ra - Rb).max() <= 1e-12 * np.abs(Ra).max()
+    monkeypatch.delenv("ORTH_REDUCE_LAUNCH")
+    monkeypatch.setenv("ORTH_SEPARATE_REDUCE", "1")
+    Qc, Rc, _ = check_against_oracle(variant, X, s)
+    assert np.array_equal(Ra, Rb) and np.array_equal(Ra, Rc)
+    assert np.array_equal(Qa, Qb) and np.array_equal(Qa, Qc)
+
+
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_one_launch_per_synchronization(variant, monkeypatch):
+    """Fused schedules synchronize inside the pass launch: no reduction
+    launch at all, one launch per pass; with ORTH_REDUCE_LAUNCH one
+    reduction launch follows each synchronization."""
+    X = I.svd_matrix(20000, 96, 1e4, seed=41)
+    say("case launches per sync", variant)
+    B.kernel_timing(True)
+    _, _, st = B.bcgs(variant, X, 16)
+    kt = B.kernel_times()
+    monkeypatch.setenv("ORTH_REDUCE_LAUNCH", "1")
+    B.kernel_timing(True)
+    _, _, st2 = B.bcgs(variant, X, 16)
+    kt2 = B.kernel_times()
+    B.kernel_timing(False)
+    assert st["syncs"] > 0 and st2["syncs"] == st["syncs"]
+    assert "reduce" not in kt and "proj" not in kt
+    assert kt["pass"]["launches"] == st["passes"]
+    assert kt2["reduce"]["launches"] == st2["syncs"]
 
 
 @pytest.mark.parametrize("variant", VARIANTS)
