@@ -111,7 +111,11 @@ int orth_bcgs_mixed_device(int variant, int precision, int64_t m, int64_t n,
  * pointers) is passed again.  tune != 0 first times the stream count and
  * workgroup share on this batch and keeps the best for batches with the
  * same shapes; it also decides between replaying captured graphs and
- * launching directly on the streams.  Batches larger than a graph-size
+ * launching directly on the streams, and picks the tile shape of the s = 16
+ * passes.  When the environment variable ORTH_TUNE_CACHE names a file, the
+ * tuned configurations are written there (one text line per device and
+ * batch shape) and read back by later processes, so that a call with
+ * tune == 0 uses a stored configuration.  Batches larger than a graph-size
  * limit are captured as several graphs launched back to back.  info (host,
  * 2 * count ints, may be NULL) receives each instance's breakdown (block,
  * pivot); config (host, 2 ints, may be NULL) the number of streams used

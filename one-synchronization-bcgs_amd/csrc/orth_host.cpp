@@ -18,8 +18,14 @@
 #include <atomic>
 #include <cmath>
 #include <climits>
+#include <fstream>
+#include <iomanip>
 #include <mutex>
+#include <sstream>
+#include <string>
 #include <vector>
+
+#include <unistd.h>
 
 #include "../../include/orth.h"
 #include "../../include/orth_ext.h"
@@ -123,7 +129,7 @@ int64_t env_bits()
                                         "ORTH_GENERIC_PASS", "ORTH_MIXED_PASS32",
                                         "ORTH_PASS_TILE32", "ORTH_PASS_LOAD8",
                                         "ORTH_SEPARATE_REDUCE", "ORTH_PASS_REGPF", "ORTH_PASS16D_ONEWG",
-                                        "ORTH_PASS16D_MINCOLS"};
+                                        "ORTH_PASS16D_MINCOLS", "ORTH_REDUCE_LAUNCH"};
     int64_t b = 0;
     for (int i = 0; i < (int)(sizeof(names) / sizeof(names[0])); i++)
         if (getenv(names[i])) b |= (int64_t)1 << i;
@@ -535,6 +541,10 @@ class Run {
     // Fraction of the device's workgroups one pass uses (the batched form
     // runs several calls side by side).
     void set_wg_scale(double f) { wg_scale_ = std::min(1.0, std::max(f, 1e-3)); }
+    // Column bound of the double-buffered s = 16 pass (0: the default).
+    void set_pass16d_min_cols(int c) { minc_ = std::max(0, c); }
+    // Synchronize inside the passes (default) or in a launch after each.
+    void set_in_pass_sync(bool on) { in_pass_sync_ = on; }
 
     // One fused pass: the pending update, then P <- A^T B (na == 0: none),
     // then the small step `after` on P when given.
@@ -590,7 +600,7 @@ class Run {
         }
         int mt = 0, nbt = 0;
         const int td = (S_ == 16 && !getenv("ORTH_PASS256") && !getenv("ORTH_MIXED_PASS32"))
-                           ? pass16d_plan(pa, &mt, &nbt, lp_first_pass())
+                           ? pass16d_plan(pa, &mt, &nbt, lp_first_pass(), minc_)
                            : 0;
         const int ts16 = td > 0 ? 32
                          : (S_ == 16 && !mx_ && !getenv("ORTH_PASS256")) ? pass16b_plan(pa, &mt, &nbt)
@@ -623,16 +633,16 @@ class Run {
         // registers and LDS allow.
         // The double-buffered pass with 16 waves holds a whole CU (one
         // workgroup).
-        const int per_cu = td == 16                          ? 1
-                           : (ts16 > 0 || mx_ == kMxFP64) ? 2
-                           : mx_                           ? 1
-                                                           : pass_wg_per_cu(S_, pa);
-        int nwg = std::max(1, (int)(std::max(1, c_.ncu) * per_cu * wg_scale_));
         // The pass synchronizes itself (reduction and small step in its last
         // workgroups) when the scratch has the ticket counters; its group
         // sums follow the partials in `part`.
-        bool tail = na > 0 && sc_.tickets && !getenv("ORTH_SEPARATE_REDUCE") &&
+        bool tail = na > 0 && sc_.tickets && in_pass_sync_ && !getenv("ORTH_SEPARATE_REDUCE") &&
                     !getenv("ORTH_REDUCE_LAUNCH");
+        const int per_cu = td == 16                          ? 1
+                           : (ts16 > 0 || mx_ == kMxFP64) ? 2
+                           : mx_                           ? 1
+                                                           : pass_wg_per_cu(S_, pa, tail);
+        int nwg = std::max(1, (int)(std::max(1, c_.ncu) * per_cu * wg_scale_));
         if (na > 0 && sc_.fixed) {
             // Fixed scratch: fewer row chunks, so that the partials fit.
             int64_t cap = (int64_t)(sc_.part_n / ((size_t)na * nb));
@@ -653,7 +663,7 @@ class Run {
         pa.part = sc_.part;
         double *Pn = na > 0 ? sc_.P[pcur_ ^ 1] : nullptr;
         if (tail) {
-            pa.tail.on = 1;
+            pa.tail_on = 1;
             pa.tail.nwg = nwg;
             pa.tail.grp = tail_group(nwg);
             pa.tail.tickets = sc_.tickets;
@@ -930,6 +940,8 @@ class Run {
     int64_t frontier_ = 0;      // columns [0, frontier_) have been written to X
     int64_t lp_from_ = -1;      // next pass: B columns from here on are reduced precision
     double wg_scale_ = 1.0;
+    int minc_ = 0;
+    bool in_pass_sync_ = true;
 
     // Debug aid (ORTH_DEBUG_DUMP set): print the reduced projection and the
     // factors after each small step.
@@ -1084,6 +1096,7 @@ struct BatchCfg {
     int streams = 4;
     double wg_scale = 0.5;
     bool graph = true;  // capture into hipGraphs (else launch directly on the streams)
+    int minc = 0;       // column bound of the double-buffered s = 16 pass (0: default)
 };
 
 // Graph nodes per captured graph at most: batches beyond it are captured as
@@ -1105,6 +1118,7 @@ struct BatchCtx {
     std::vector<int64_t> key;           // batch the graphs were captured for
     std::vector<int64_t> nocap;         // batch whose capture failed (runs directly)
     std::vector<std::pair<std::vector<int64_t>, BatchCfg>> tuned;  // shape signature -> config
+    bool cache_read = false;  // the tuning cache file was read for this device
 };
 
 BatchCtx g_batch[kMaxDevices];
@@ -1171,6 +1185,7 @@ std::vector<int64_t> full_key(const BatchArgs &a, const BatchCfg &cfg)
     k.push_back(cfg.graph ? 1 : 0);
     k.push_back(cfg.streams);
     k.push_back((int64_t)(cfg.wg_scale * 1e6));
+    k.push_back(cfg.minc);
     // Captured graphs hold the small steps' arguments, the shift among them,
     // and the kernels the environment switches selected.
     const double ps = g_pshift.load();
@@ -1286,6 +1301,13 @@ bool batch_enqueue(BatchCtx &b, const BatchArgs &a, const BatchCfg &cfg,
         for (int i : lists[k]) {
             Run r(b.slot[k], b.slot[k].sc, a.m[i], a.n[i], a.s[i], a.X[i], a.ldX[i], st);
             r.set_wg_scale(cfg.wg_scale);
+            r.set_pass16d_min_cols(cfg.minc);
+            // Concurrent instances synchronize in a launch after each pass:
+            // inside the passes, every workgroup's device-scope release and
+            // the last workgroups' acquires reach into the L2 the other
+            // streams' instances are using (the 512-instance batch ran 8 %
+            // (P-1S) and 21 % (BCGSI+) slower that way).
+            r.set_in_pass_sync(false);
             if (a.mx) r.set_mixed(a.mx, a.X32[i], a.ldX32[i]);
             good = r.prepare() && r.run(a.variant) &&
                    r.copy_R_out(a.R[i], a.ldR[i], hipMemcpyDeviceToDevice) &&
@@ -1441,6 +1463,115 @@ struct InputBackup {
     }
 };
 
+// Persisted tuning: when ORTH_TUNE_CACHE names a file, the tuner's results
+// are kept there across processes, one line per device and batch signature:
+//   orth-tune-1 <device> <signature ...> : <streams> <wg_scale> <graph> <minc>
+// (<device>: architecture and CU count; <signature>: variant, mixed
+// precision, count, then m n s of every instance).  The file is read on the
+// first batched call on a device and rewritten whole (a temporary file
+// renamed over it) after each tuning; other devices' lines are kept, and
+// lines that do not parse are dropped.
+const char *const kTuneTag = "orth-tune-1";
+
+std::string tune_device_id(int dev)
+{
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, dev) != hipSuccess) {
+        (void)hipGetLastError();
+        return "unknown";
+    }
+    std::string arch = p.gcnArchName;
+    const size_t colon = arch.find(':');  // target features
+    if (colon != std::string::npos) arch.resize(colon);
+    for (char &ch : arch)
+        if (ch == ' ') ch = '_';
+    return arch + "/" + std::to_string(p.multiProcessorCount);
+}
+
+bool parse_tune_line(const std::string &line, std::string *dev, std::vector<int64_t> *sig,
+                     BatchCfg *cfg)
+{
+    std::istringstream in(line);
+    std::string tag, tok;
+    if (!(in >> tag) || tag != kTuneTag || !(in >> *dev)) return false;
+    sig->clear();
+    while (in >> tok && tok != ":") {
+        char *end = nullptr;
+        const long long v = strtoll(tok.c_str(), &end, 10);
+        if (!end || *end) return false;
+        sig->push_back(v);
+    }
+    int graph = -1;
+    if (tok != ":" || !(in >> cfg->streams >> cfg->wg_scale >> graph >> cfg->minc)) return false;
+    cfg->graph = graph == 1;
+    return sig->size() >= 3 && (*sig)[2] >= 1 && sig->size() == 3 + 3 * (size_t)(*sig)[2] &&
+           cfg->streams >= 1 && cfg->streams <= kMaxSlots && cfg->wg_scale > 0.0 &&
+           cfg->wg_scale <= 1.0 && (graph == 0 || graph == 1) && cfg->minc >= 0;
+}
+
+std::string tune_line(const std::string &dev, const std::vector<int64_t> &sig, const BatchCfg &c)
+{
+    std::ostringstream o;
+    o << kTuneTag << ' ' << dev;
+    for (int64_t v : sig) o << ' ' << v;
+    o << " : " << c.streams << ' ' << std::setprecision(17) << c.wg_scale << ' '
+      << (c.graph ? 1 : 0) << ' ' << c.minc;
+    return o.str();
+}
+
+void tune_cache_read(BatchCtx &b, int dev)
+{
+    if (b.cache_read) return;
+    b.cache_read = true;
+    const char *path = getenv("ORTH_TUNE_CACHE");
+    if (!path || !*path) return;
+    std::ifstream f(path);
+    const std::string me = tune_device_id(dev);
+    std::string line, d;
+    std::vector<int64_t> sig;
+    BatchCfg c;
+    while (f && std::getline(f, line)) {
+        if (!parse_tune_line(line, &d, &sig, &c) || d != me) continue;
+        bool known = false;
+        for (const auto &t : b.tuned) known = known || t.first == sig;
+        if (!known) b.tuned.push_back({sig, c});
+    }
+}
+
+void tune_cache_write(const BatchCtx &b, int dev)
+{
+    const char *path = getenv("ORTH_TUNE_CACHE");
+    if (!path || !*path) return;
+    const std::string me = tune_device_id(dev);
+    std::vector<std::string> lines;
+    {
+        std::ifstream f(path);
+        std::string line, d;
+        std::vector<int64_t> sig;
+        BatchCfg c;
+        while (f && std::getline(f, line)) {
+            if (!parse_tune_line(line, &d, &sig, &c)) continue;
+            bool mine = false;
+            if (d == me)
+                for (const auto &t : b.tuned) mine = mine || t.first == sig;
+            if (!mine) lines.push_back(line);
+        }
+    }
+    for (const auto &t : b.tuned) lines.push_back(tune_line(me, t.first, t.second));
+    const std::string tmp = std::string(path) + ".tmp" + std::to_string((long long)getpid());
+    bool good = false;
+    {
+        std::ofstream o(tmp, std::ios::trunc);
+        for (const auto &l : lines) o << l << '\n';
+        o.flush();
+        good = o.good();
+    }
+    if (!good || rename(tmp.c_str(), path) != 0) {
+        fprintf(stderr, "orth: could not write the tuning cache %s\n", path);
+        (void)remove(tmp.c_str());
+    }
+}
+
 BatchCfg batch_cfg(const BatchCtx &b, const BatchArgs &a)
 {
     const std::vector<int64_t> sig = shape_sig(a);
@@ -1499,6 +1630,23 @@ bool batch_tune(BatchCtx &b, const BatchArgs &a, const InputBackup &bk, BatchCfg
             }
         }
     }
+    // Tile shape of the s = 16 passes, with the best placement: the
+    // double-buffered 32-row pass on every slab of 16 columns or more, or
+    // never (the 512-thread super-tile pass throughout), against the
+    // library's column bound.
+    for (const int mc : {16, 1 << 20}) {
+        if (!good) break;
+        BatchCfg cand = bestc;
+        cand.minc = mc;
+        good = bk.restore(a, b.main) && batch_run(b, a, cand) && bk.restore(a, b.main) &&
+               ok(hipEventRecord(e0, b.main)) && batch_run(b, a, cand) &&
+               ok(hipEventRecord(e1, b.main)) && ok(hipEventSynchronize(e1));
+        float ms = 0.f;
+        if (good && ok(hipEventElapsedTime(&ms, e0, e1)) && ms < best) {
+            best = ms;
+            bestc = cand;
+        }
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     *best_out = bestc;
@@ -1515,6 +1663,7 @@ int batched_entry(const BatchArgs &a, int *info, void *stream, int tune, int *cf
     if (!ctx_init(c)) return 2;
     BatchCtx &b = g_batch[dev];
     if (b.broken || !batch_init(b)) return 2;
+    tune_cache_read(b, dev);
     // Order after the caller's stream and the library's earlier calls.
     if (!ok(hipStreamSynchronize((hipStream_t)stream))) return 2;
     if (c.done_valid && !ok(hipEventSynchronize(c.done))) return 2;
@@ -1534,6 +1683,7 @@ int batched_entry(const BatchArgs &a, int *info, void *stream, int tune, int *cf
                     found = true;
                 }
             if (!found) b.tuned.push_back({sig, best});
+            tune_cache_write(b, dev);
             cfg = best;
             if (!bk.restore(a, b.main)) return 2;
         } else if (tl_hip_error) {

@@ -105,10 +105,10 @@ inline int tail_group(int nwg)
 // last of those sums the group sums into out and runs the small step.  The
 // counters are set back to zero by the workgroups that read them last.
 struct TailArgs {
-    int on;        // 1: the pass reduces its own partials
-    int small;     // 1: then runs `sm` (sm.P = out) in its last workgroup
+    int small;     // 1: after the sum, run `sm` (sm.P = out) in the last workgroup
     int nwg;       // workgroups of the pass
     int grp;       // tail_group(nwg)
+    int pad;
     int *tickets;  // [0, kTailMaxGroups) group counters, [kTailTop] the top counter
     double *lvl;   // group sums: ceil(nwg / grp) x (na nb) doubles
     double *out;   // the sum, na x nb, ld na
@@ -169,6 +169,10 @@ struct PassArgs {
     double *X;
     int64_t ldX;
     int64_t m;
+    // 1: the pass synchronizes itself (tail below).  Beside the fields every
+    // workgroup reads at its start, so that the check at its end finds the
+    // line cached.
+    int tail_on;
     int ncols;  // columns of X staged per tile
     int wl;     // LDS tile width: multiple of 16 covering every range
     int k1;     // inner dimension of step 1 (0: no product)
@@ -201,8 +205,9 @@ constexpr int kPassMaxCols = 288;
 // LDS bytes the fused pass needs, or 0 when the shape does not fit it.
 size_t pass_lds_bytes(const PassArgs &a, int F = 1);
 hipError_t launch_pass(int S, const PassArgs &a, int nwg, hipStream_t st);
-// Workgroups per CU the generic pass for this shape can keep resident.
-int pass_wg_per_cu(int S, const PassArgs &a);
+// Workgroups per CU the generic pass for this shape can keep resident (tail:
+// the pass runs its synchronization, whose LDS it then holds).
+int pass_wg_per_cu(int S, const PassArgs &a, bool tail);
 // Rows per tile (the row-chunk unit) of the 256-thread passes for a.
 int pass_rows(int S, const PassArgs &a);
 
@@ -217,8 +222,10 @@ hipError_t launch_pass16b(const PassArgs &a, int nwg, hipStream_t st);
 // (rows_per must be a multiple of 32): waves per workgroup (8 or 16), or 0
 // when it does not apply.
 // any_width: also below the column bound (the mixed form with reduced
-// precisions, whose 512-thread pass runs one workgroup per CU).
-int pass16d_plan(const PassArgs &a, int *mt, int *nb, bool any_width = false);
+// precisions, whose 512-thread pass runs one workgroup per CU).  min_cols:
+// the column bound (0: the library's default; the batched form's tuner
+// picks it per batch).
+int pass16d_plan(const PassArgs &a, int *mt, int *nb, bool any_width = false, int min_cols = 0);
 // Launch with the plan's waves per workgroup (nw) and N-tiles (nb).
 hipError_t launch_pass16d(const PassArgs &a, int nw, int nb, int nwg, hipStream_t st);
 // Its mixed form (a.X32 set): mx = kMxFP64 (FP32 storage, FP64 products) or

@@ -1088,7 +1088,7 @@ hipError_t launch_pass16b(const PassArgs &a, int nwg, hipStream_t st)
 // to 256-row super-tiles (ORTH_PASS16D_MINCOLS overrides the bound).
 constexpr int kPass16dMinCols = 128;
 
-int pass16d_plan(const PassArgs &a, int *mt, int *nb, bool any_width)
+int pass16d_plan(const PassArgs &a, int *mt, int *nb, bool any_width, int min_cols)
 {
     if (getenv("ORTH_PASS_REGPF")) return 0;
     if (a.n1 != 32 || a.seq || (a.na > 0 && a.nb > 32)) return 0;
@@ -1103,7 +1103,7 @@ int pass16d_plan(const PassArgs &a, int *mt, int *nb, bool any_width)
     // Row pairs are loaded whole: an odd row count would read past the last
     // row of the last column.
     if (a.ldX % 2 != 0 || a.m % 2 != 0 || (uintptr_t)a.X % 16 != 0) return 0;
-    int minc = any_width ? 1 : kPass16dMinCols;
+    int minc = any_width ? 1 : min_cols > 0 ? min_cols : kPass16dMinCols;
     if (const char *e = getenv("ORTH_PASS16D_MINCOLS")) minc = atoi(e);
     if (a.ncols < minc || a.ncols > a.wl) return 0;
     const bool force16 = getenv("ORTH_PASS16D_ONEWG") != nullptr;
@@ -1284,7 +1284,7 @@ hipError_t launch_pass_t(const PassArgs &a, int nwg, hipStream_t st)
     return launch_pass_v<S, kPassMaxTiles, kPF>(a, lds, nwg, st);
 }
 
-int pass_wg_per_cu(int S, const PassArgs &a)
+int pass_wg_per_cu(int S, const PassArgs &a, bool tail)
 {
     const size_t lds = pass_lds_bytes(a, pass_f(S, a));
     if (lds == 0) return 1;
@@ -1293,13 +1293,14 @@ int pass_wg_per_cu(int S, const PassArgs &a)
     const bool p16 = S == 16 && pass16_mt(a) > 0 && !getenv("ORTH_GENERIC_PASS");
     const int by_regs = p16 ? 1 : pass_narrow(a) ? pass_waves(S, 4) : pass_waves(S, kPassMaxTiles);
     // The in-pass tail raises small LDS footprints to its own.
-    const int by_lds = (int)std::max<size_t>(1, (160 * 1024) / std::max(lds, kTailLdsBytes));
+    const int by_lds =
+        (int)std::max<size_t>(1, (160 * 1024) / (tail ? std::max(lds, kTailLdsBytes) : lds));
     return std::max(1, std::min(by_regs, by_lds));
 }
 
 size_t pass_tail_lds(const PassArgs &a, size_t lds)
 {
-    return a.tail.on ? std::max(lds, kTailLdsBytes) : lds;
+    return a.tail_on ? std::max(lds, kTailLdsBytes) : lds;
 }
 
 hipError_t launch_pass(int S, const PassArgs &a, int nwg, hipStream_t st)

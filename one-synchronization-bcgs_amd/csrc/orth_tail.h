@@ -19,23 +19,34 @@ constexpr size_t kTailLdsBytes = (size_t)(kSmallLds + 1) * sizeof(double);
 
 // Sum of the partials [k0, k1) of element e in workgroup order, starting
 // from the first (no added zero, so the sign of an exact zero is kept).
-// Eight loads are issued before their additions.
+// The loads of W partials are issued together (indices past the range are
+// clamped to its last partial and their values not added), then added in
+// order: one memory round trip per W partials.
+template <int W>
+static __device__ __forceinline__ double part_range_sum_w(const double *p, int64_t E, int n)
+{
+    double s = 0.0;
+    for (int b = 0; b < n; b += W) {
+        double v[W];
+#pragma unroll
+        for (int i = 0; i < W; i++) v[i] = p[(int64_t)min(b + i, n - 1) * E];
+#pragma unroll
+        for (int i = 0; i < W; i++) {
+            const int k = b + i;
+            s = k == 0 ? v[i] : k < n ? s + v[i] : s;
+        }
+    }
+    return s;
+}
+
 static __device__ __forceinline__ double part_range_sum(const double *part, int64_t E, int64_t e,
                                                         int k0, int k1)
 {
     const double *p = part + (int64_t)k0 * E + e;
     const int n = k1 - k0;
-    double s = p[0];
-    int k = 1;
-    for (; k + 7 < n; k += 8) {
-        double v[8];
-#pragma unroll
-        for (int i = 0; i < 8; i++) v[i] = p[(int64_t)(k + i) * E];
-#pragma unroll
-        for (int i = 0; i < 8; i++) s += v[i];
-    }
-    for (; k < n; k++) s += p[(int64_t)k * E];
-    return s;
+    if (n <= 8) return part_range_sum_w<8>(p, E, n);
+    if (n <= 16) return part_range_sum_w<16>(p, E, n);
+    return part_range_sum_w<32>(p, E, n);
 }
 
 // Publish what this workgroup stored and take a ticket of `counter`
@@ -70,17 +81,24 @@ static __device__ __forceinline__ uint64_t kernarg_stash()
     return p;
 }
 
-// The tail's arguments, read from the kernel's argument segment (PassArgs
-// is each pass kernel's only argument, at offset 0) at the tail, so that
-// none of them occupies a register during the pass.
-static __device__ __forceinline__ TailArgs tail_args(uint64_t stash)
+// The kernel's argument segment (PassArgs is each pass kernel's only
+// argument, at offset 0) from the stashed address; reads through it stay at
+// the tail, so that none of the tail's arguments occupies a register
+// during the pass.
+typedef const char __attribute__((address_space(4))) *kernarg_ptr;
+static __device__ __forceinline__ kernarg_ptr kernarg_base(uint64_t stash)
+{
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)stash);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(stash >> 32));
+    return (kernarg_ptr)((uint64_t)hi << 32 | lo);
+}
+
+static __device__ __forceinline__ TailArgs tail_args(kernarg_ptr kp)
 {
     static_assert(sizeof(TailArgs) % 8 == 0 && offsetof(PassArgs, tail) % 8 == 0,
                   "tail arguments are copied as 8-byte words");
     typedef const uint64_t __attribute__((address_space(4))) *wptr;
-    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)stash);
-    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(stash >> 32));
-    const wptr src = (wptr)(((uint64_t)hi << 32 | lo) + offsetof(PassArgs, tail));
+    const wptr src = (wptr)(kp + offsetof(PassArgs, tail));
     uint64_t w[sizeof(TailArgs) / 8];
 #pragma unroll
     for (int i = 0; i < (int)(sizeof(TailArgs) / 8); i++) w[i] = src[i];
@@ -91,7 +109,7 @@ static __device__ __forceinline__ TailArgs tail_args(uint64_t stash)
 
 // Every thread of every workgroup of the pass calls this after its
 // workgroup's partial is stored; it returns at once unless the pass
-// synchronizes itself (TailArgs::on).  stash: kernarg_stash() from the
+// synchronizes itself (PassArgs::tail_on).  stash: kernarg_stash() from the
 // kernel entry; nt: the workgroup's thread count (block size and grid size
 // come from the arguments, so that nothing of the dispatch stays live
 // through the pass).  sm: the pass's dynamic LDS, at least pass_tail_lds()
@@ -99,8 +117,10 @@ static __device__ __forceinline__ TailArgs tail_args(uint64_t stash)
 static __device__ __forceinline__ void pass_tail(const PassArgs &a, double *sm, uint64_t stash,
                                                  int nt)
 {
-    const TailArgs t = tail_args(stash);
-    if (!t.on) return;
+    const kernarg_ptr kp = kernarg_base(stash);
+    typedef const int __attribute__((address_space(4))) *iptr;
+    if (*(iptr)(kp + offsetof(PassArgs, tail_on)) == 0) return;
+    const TailArgs t = tail_args(kp);
     const int nwg = t.nwg;
     const int64_t E = (int64_t)a.na * a.nb;
     const int ngrp = (nwg + t.grp - 1) / t.grp;

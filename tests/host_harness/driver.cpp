@@ -90,13 +90,17 @@ int main(int argc, char **argv)
             expect(what, rc, 0);
         }
     };
+    // HARNESS_NO_TUNE: the first call of each batch does not tune (its
+    // configuration comes from the library's table, or its tuning cache).
+    const int tune = getenv("HARNESS_NO_TUNE") ? 0 : 1;
     for (int variant : {ORTH_P1S, ORTH_I2, ORTH_PIPI}) {
         Batch b = make(count, 7 + variant, false, mmax);
         std::vector<int> info(2 * count);
         int cfg[3] = {0, 0, 0};
         check("batched fp64: tune", orth_bcgs_batched_device(
                   variant, count, b.m.data(), b.n.data(), b.s.data(), b.X.data(), b.ldX.data(),
-                  b.R.data(), b.ldR.data(), info.data(), 1, cfg, st));
+                  b.R.data(), b.ldR.data(), info.data(), tune, cfg, st));
+        printf("batched fp64 variant %d config: %d %d\n", variant, cfg[0], cfg[1]);
         for (int rep = 0; rep < 3; rep++)
             check("batched fp64: replay", orth_bcgs_batched_device(
                       variant, count, b.m.data(), b.n.data(), b.s.data(), b.X.data(),

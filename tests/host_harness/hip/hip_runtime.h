@@ -51,6 +51,12 @@ typedef struct hipPointerAttribute_t {
 
 typedef enum hipDeviceAttribute_t { hipDeviceAttributeMultiprocessorCount = 1 } hipDeviceAttribute_t;
 
+typedef struct hipDeviceProp_t {
+    char name[256];
+    char gcnArchName[256];
+    int multiProcessorCount;
+} hipDeviceProp_t;
+
 typedef enum hipStreamCaptureMode {
     hipStreamCaptureModeGlobal = 0,
     hipStreamCaptureModeThreadLocal = 1,
@@ -106,6 +112,7 @@ hipError_t hipGraphDestroy(hipGraph_t g);
 hipError_t hipGetLastError(void);
 hipError_t hipGetDevice(int *d);
 hipError_t hipDeviceGetAttribute(int *v, hipDeviceAttribute_t a, int dev);
+hipError_t hipGetDeviceProperties(hipDeviceProp_t *p, int dev);
 hipError_t hipDeviceSynchronize(void);
 hipError_t hipPointerGetAttributes(hipPointerAttribute_t *a, const void *p);
 const char *hipGetErrorString(hipError_t e);

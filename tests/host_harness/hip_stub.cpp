@@ -315,6 +315,14 @@ hipError_t hipDeviceGetAttribute(int *v, hipDeviceAttribute_t, int)
     *v = 256;
     return hipSuccess;
 }
+hipError_t hipGetDeviceProperties(hipDeviceProp_t *p, int)
+{
+    memset(p, 0, sizeof(*p));
+    strcpy(p->name, "stand-in");
+    strcpy(p->gcnArchName, "gfx950:sramecc+:xnack-");
+    p->multiProcessorCount = 256;
+    return hipSuccess;
+}
 hipError_t hipDeviceSynchronize(void)
 {
     std::lock_guard<std::mutex> lk(g_mu);
@@ -386,7 +394,7 @@ hipError_t launch_pass(int, const PassArgs &, int, hipStream_t st)
 {
     return stub_launch(st);
 }
-int pass_wg_per_cu(int, const PassArgs &)
+int pass_wg_per_cu(int, const PassArgs &, bool)
 {
     return 2;
 }
@@ -411,9 +419,10 @@ hipError_t launch_pass16b(const PassArgs &, int, hipStream_t st)
 {
     return stub_launch(st);
 }
-int pass16d_plan(const PassArgs &a, int *mt, int *nb, bool any_width)
+int pass16d_plan(const PassArgs &a, int *mt, int *nb, bool any_width, int min_cols)
 {
-    if (a.n1 != 32 || a.seq || a.ncols < (any_width ? 1 : 128) || a.wl > 256) return 0;
+    const int minc = any_width ? 1 : min_cols > 0 ? min_cols : 128;
+    if (a.n1 != 32 || a.seq || a.ncols < minc || a.wl > 256) return 0;
     *mt = 1;
     *nb = a.na > 0 ? (a.nb + 15) / 16 : 1;
     return a.wl <= 128 ? 8 : 16;
