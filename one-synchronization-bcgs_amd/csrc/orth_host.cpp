@@ -37,6 +37,10 @@ namespace {
 
 constexpr int kMaxDevices = 64;
 constexpr int kProjTargetWG = 1024;
+// Rows per CU from which a fused pass synchronizes inside its own launch
+// (2^23 rows on 256 CUs); shorter slabs end each pass with one
+// k_reduce_small launch.
+constexpr int64_t kTailMinRowsPerCU = 32768;
 
 struct DevCtx {
     std::mutex mu;
@@ -129,7 +133,8 @@ int64_t env_bits()
                                         "ORTH_GENERIC_PASS", "ORTH_MIXED_PASS32",
                                         "ORTH_PASS_TILE32", "ORTH_PASS_LOAD8",
                                         "ORTH_SEPARATE_REDUCE", "ORTH_PASS_REGPF", "ORTH_PASS16D_ONEWG",
-                                        "ORTH_PASS16D_MINCOLS", "ORTH_REDUCE_LAUNCH"};
+                                        "ORTH_PASS16D_MINCOLS", "ORTH_REDUCE_LAUNCH",
+                                        "ORTH_IN_PASS_SYNC"};
     int64_t b = 0;
     for (int i = 0; i < (int)(sizeof(names) / sizeof(names[0])); i++)
         if (getenv(names[i])) b |= (int64_t)1 << i;
@@ -634,10 +639,16 @@ class Run {
         // The double-buffered pass with 16 waves holds a whole CU (one
         // workgroup).
         // The pass synchronizes itself (reduction and small step in its last
-        // workgroups) when the scratch has the ticket counters; its group
-        // sums follow the partials in `part`.
+        // workgroups) when the scratch has the ticket counters and the slab
+        // is long: the tail puts device-scope cache maintenance (each
+        // workgroup's release, the last workgroups' acquires at two levels)
+        // on the critical path, 4-15 us per synchronization more than a
+        // launch boundary, which only passes of kTailMinRowsPerCU rows per CU
+        // and more hide.  Its group sums follow the partials in `part`.
         bool tail = na > 0 && sc_.tickets && in_pass_sync_ && !getenv("ORTH_SEPARATE_REDUCE") &&
-                    !getenv("ORTH_REDUCE_LAUNCH");
+                    !getenv("ORTH_REDUCE_LAUNCH") &&
+                    (m_ >= kTailMinRowsPerCU * (int64_t)std::max(1, c_.ncu) ||
+                     getenv("ORTH_IN_PASS_SYNC"));
         const int per_cu = td == 16                          ? 1
                            : (ts16 > 0 || mx_ == kMxFP64) ? 2
                            : mx_                           ? 1

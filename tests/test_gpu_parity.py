@@ -324,7 +324,10 @@ def test_separate_reduction_matches_in_kernel(variant, shape, monkeypatch):
     thread in a fixed order whatever the workgroup size)."""
     m, n, s = shape
     X = I.svd_matrix(m, n, 1e4, seed=33)
+    # In-pass synchronization also below its default row bound.
+    monkeypatch.setenv("ORTH_IN_PASS_SYNC", "1")
     Qa, Ra, _ = check_against_oracle(variant, X, s)
+    monkeypatch.delenv("ORTH_IN_PASS_SYNC")
     monkeypatch.setenv("ORTH_REDUCE_LAUNCH", "1")
     Qb, Rb, _ = check_against_oracle(variant, X, s)
     monkeypatch.delenv("ORTH_REDUCE_LAUNCH")
@@ -336,14 +339,17 @@ def test_separate_reduction_matches_in_kernel(variant, shape, monkeypatch):
 
 @pytest.mark.parametrize("variant", VARIANTS)
 def test_one_launch_per_synchronization(variant, monkeypatch):
-    """Fused schedules synchronize inside the pass launch: no reduction
-    launch at all, one launch per pass; with ORTH_REDUCE_LAUNCH one
-    reduction launch follows each synchronization."""
+    """Fused schedules synchronize inside the pass launch (forced below the
+    default row bound): no reduction launch at all, one launch per pass;
+    with ORTH_REDUCE_LAUNCH one reduction launch follows each
+    synchronization."""
     X = I.svd_matrix(20000, 96, 1e4, seed=41)
     say("case launches per sync", variant)
+    monkeypatch.setenv("ORTH_IN_PASS_SYNC", "1")
     B.kernel_timing(True)
     _, _, st = B.bcgs(variant, X, 16)
     kt = B.kernel_times()
+    monkeypatch.delenv("ORTH_IN_PASS_SYNC")
     monkeypatch.setenv("ORTH_REDUCE_LAUNCH", "1")
     B.kernel_timing(True)
     _, _, st2 = B.bcgs(variant, X, 16)

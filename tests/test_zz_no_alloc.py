@@ -60,8 +60,14 @@ def run_noalloc(variant, X, s):
                                                                                  int(inf[1]))
 
 
+@pytest.mark.parametrize("in_pass", [False, True])
 @pytest.mark.parametrize("variant", list(B.VARIANTS))
-def test_noalloc_matches_device_entry(variant):
+def test_noalloc_matches_device_entry(variant, in_pass, monkeypatch):
+    # in_pass: the passes synchronize inside their launch also at these
+    # lengths (on the fixed scratch the group sums take room from the
+    # partials, so the passes use fewer row chunks).
+    if in_pass:
+        monkeypatch.setenv("ORTH_IN_PASS_SYNC", "1")
     # The second shape is smaller in every dimension but s.
     for m, n, s, kappa in ((120000, 256, 16, 1e6), (30001, 96, 8, 1e3), (4096, 40, 16, 1e10)):
         X = I.svd_matrix(m, n, kappa, seed=41)
